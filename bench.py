@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Headline benchmark: FP64-accurate TFLOP/s at N=16384, split-precision MFMA
+vs native FP64 MFMA, on one MI355X.
+
+Workload (one step): one rank's local product of the 2D SUMMA 2x2 grid at
+N=16384, C_ij (8192 x 8192) = A_ik (8192 x 16384) . B_kj (16384 x 8192), FP64,
+inputs uniform[-1,1) from the splitmix64 stream (A seed 1, B seed 2), all
+operands resident in HBM.  Both forms are credited F = 2 M N K flop.
+
+  * split (the `value`): libmgx INT8 Ozaki product with `--slices` slices,
+    timed end to end (exponent scan, slicing, all slice-pair products,
+    reconstruction);
+  * plain: libmgx FP64 MFMA product on the same inputs.
+
+The dominant kernel of the split form is the fused slice-pair product; its
+duration is read from HIP events recorded on the launch stream around that
+kernel inside the timed region.  The CPU baseline is the oracle's
+restatement of the reference kernel's arithmetic (sequential fma chains)
+run with OpenMP on the host cores over a row slab of the same product.
+
+Prints ONE JSON line on stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+METRIC = "FP64-accurate TFLOP/s at N=16384: split-precision MFMA vs native FP64 MFMA"
+
+
+def parse_args():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--slices", type=int, default=8)
+    ap.add_argument("--m", type=int, default=8192)
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--k", type=int, default=16384)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="target CPU time of the CPU-baseline row slab")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(m: int, n: int, k: int, target_s: float) -> dict:
+    """Oracle port of the reference product, timed on a row slab (host cores)."""
+    import oracle  # test infrastructure: used only as the CPU baseline
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    B = oracle.fill(k, n, 1, seed=2)
+
+    def run(rows: int) -> float:
+        A = oracle.fill(rows, k, 1, seed=1)  # the first `rows` rows of A
+        t0 = time.perf_counter()
+        oracle.gemm_kernel_order(A, B, nthreads=threads)
+        return time.perf_counter() - t0
+
+    r0 = 8
+    t0 = run(r0)
+    rows = int(max(8, min(m, r0 * target_s / max(t0, 1e-3))))
+    rows = max(8, rows // 8 * 8)
+    t = run(rows)
+    tflops = 2.0 * rows * n * k / t / 1e12
+    full_s = 2.0 * m * n * k / (tflops * 1e12)
+    return {
+        "value": tflops,
+        "unit": "TFLOP/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{rows} rows x {n} cols x K={k} of the {m}x{n}x{k} share "
+                   f"({t:.2f} s); reference kernel arithmetic (sequential fma chains), "
+                   f"OpenMP {threads} threads; full share extrapolated {full_s:.1f} s"),
+    }
+
+
+def main() -> int:
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU's local product; --gpus {args.gpus} is not supported "
+              "(the multi-GPU exchange is not part of this build)", file=sys.stderr)
+        return 2
+    import torch
+    import mgxbench_amd as mgx
+
+    m, n, k, s = args.m, args.n, args.k, args.slices
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    log(f"allocating m={m} n={n} k={k} slices={s}")
+    A = torch.empty((m, k), dtype=torch.float64, device=dev)
+    B = torch.empty((k, n), dtype=torch.float64, device=dev)
+    C = torch.empty((m, n), dtype=torch.float64, device=dev)
+    Cp = torch.empty((m, n), dtype=torch.float64, device=dev)
+    mgx.fill(A, m, k, k, 1, seed=1)
+    mgx.fill(B, k, n, n, 1, seed=2)
+    ws_bytes = mgx.split_workspace_bytes(mgx.FORM_INT8, s, m, n, k)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+    F = 2.0 * m * n * k
+
+    def plain_step():
+        mgx.ABMultiply(A, B, Cp, m, n, k, k, n, n, 32, stream=stream)
+
+    def split_step(ev=None):
+        if ev is None:
+            mgx.split_gemm(A, B, C, m, n, k, k, n, n, slices=s, workspace=ws,
+                           workspace_bytes=ws_bytes, stream=stream)
+        else:
+            mgx.split_gemm_events(A, B, C, m, n, k, k, n, n, ev, slices=s, workspace=ws,
+                                  workspace_bytes=ws_bytes, stream=stream)
+
+    def timed(step, steps):
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0.record(stream)
+        for i in range(steps):
+            step(i)
+        t1.record(stream)
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) / steps
+
+    # plain FP64 MFMA
+    log("plain warmup")
+    for _ in range(args.warmup):
+        plain_step()
+    torch.cuda.synchronize()
+    log("plain timed")
+    ms_plain = timed(lambda i: plain_step(), args.steps)
+    log(f"plain {ms_plain:.3f} ms/step")
+
+    # split precision, with phase events inside the timed region
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    for evs in events:  # materialise the HIP events before the timed region
+        for e in evs:
+            e.record(stream)
+    log("split warmup")
+    for _ in range(args.warmup):
+        split_step()
+    torch.cuda.synchronize()
+    log("split timed")
+    ms_split = timed(lambda i: split_step(events[i]), args.steps)
+    gemm_ms = sum(ev[2].elapsed_time(ev[3]) for ev in events) / args.steps
+    scan_ms = sum(ev[0].elapsed_time(ev[1]) for ev in events) / args.steps
+    slice_ms = sum(ev[1].elapsed_time(ev[2]) for ev in events) / args.steps
+    log(f"split {ms_split:.3f} ms/step (scan {scan_ms:.3f}, slice {slice_ms:.3f}, "
+        f"product {gemm_ms:.3f})")
+
+    # agreement of the two forms on the full share (both FP64-accurate)
+    diff = float(torch.linalg.vector_norm(C - Cp) / torch.linalg.vector_norm(Cp))
+
+    tf_split = F / (ms_split * 1e-3) / 1e12
+    tf_plain = F / (ms_plain * 1e-3) / 1e12
+    pairs = s * (s + 1) // 2
+    int8_ops = pairs * F
+    achieved = int8_ops / (gemm_ms * 1e-3) / 1e12
+    peak_i8 = mgx.peak(mgx.FORM_INT8)
+    out = {
+        "metric": METRIC,
+        "value": round(tf_split, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_split, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"2D SUMMA 2x2 rank share at N=16384: C {m}x{n} += A {m}x{k} . "
+                         f"B {k}x{n}, FP64 uniform[-1,1), HBM-resident"),
+            "m": m, "n": n, "k": k,
+            "form": f"int8 Ozaki split, s={s} slices ({pairs} slice-pair products)",
+            "parallelism": "single GPU",
+        },
+        "plain_fp64": {
+            "tflops": round(tf_plain, 3),
+            "ms_per_step": round(ms_plain, 4),
+            "frac_of_fp64_mfma_peak": round(tf_plain / mgx.peak(mgx.FORM_FP64), 4),
+        },
+        "speedup_vs_plain": round(tf_split / tf_plain, 4),
+        "split_phase_ms": {"exponent_scan": round(scan_ms, 4), "slicing": round(slice_ms, 4),
+                           "slice_pair_product": round(gemm_ms, 4)},
+        "split_vs_plain_rel_fro": diff,
+        "roofline": {
+            "kernel": "ozaki_int8_gemm (fused slice-pair product)",
+            "bound": "mfma",
+            "achieved": round(achieved, 2),
+            "peak": peak_i8,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / peak_i8, 4),
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(m, n, k, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

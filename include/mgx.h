@@ -1,0 +1,159 @@
+/*
+ * mgx.h -- C ABI of libmgx, the MI355X (gfx950) local FP64 block product.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and
+ * returns an int status (MGX_OK == 0) instead of aborting the process.
+ * Matrices are row-major with explicit leading dimensions (in elements).
+ * Device pointers are owned by the caller unless stated otherwise.
+ * `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *
+ * Threading: entry points may be called from any host thread; calls that
+ * share a stream are ordered by that stream.  The library keeps per-device
+ * scratch allocations (validation partials, split-form workspaces grown on
+ * demand) that are reused across calls on the same device: two calls that
+ * use the library-owned split workspace on one device must be ordered on
+ * one stream (they race otherwise).
+ */
+#ifndef MGX_H
+#define MGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    MGX_OK = 0,
+    MGX_ERR_INVALID_ARG = 1,   /* a size, leading dimension or pointer is invalid */
+    MGX_ERR_HIP = 2,           /* a HIP runtime call failed (see mgx_last_error) */
+    MGX_ERR_UNSUPPORTED = 3,   /* form / slice count / option not supported */
+    MGX_ERR_WORKSPACE = 4,     /* caller workspace too small */
+    MGX_ERR_ALLOC = 5          /* device or pinned allocation failed */
+};
+
+/* Split-precision forms. */
+enum {
+    MGX_FORM_FP64 = 0,         /* plain FP64 MFMA (v_mfma_f64_16x16x4_f64) */
+    MGX_FORM_INT8 = 1,         /* Ozaki split: INT8 slices, exact INT32 MFMA accumulation */
+    MGX_FORM_BF16 = 2          /* Ozaki split: BF16 slices, exact FP32 MFMA accumulation */
+};
+
+/* Human-readable text of the last error on the calling thread. */
+const char *mgx_last_error(void);
+/* Library version string, e.g. "0.1.0". */
+const char *mgx_version(void);
+
+/* Published dense peak of the MI355X resource named by `form`, in TFLOP/s
+ * (TOP/s for INT8): FP64 MFMA 78.6, INT8 5000, BF16 2500; form = -1 gives
+ * HBM bandwidth in GB/s (8000). Returns a negative value for unknown forms. */
+double mgx_peak(int form);
+
+/* ---- plain FP64 GEMM ---------------------------------------------------
+ * Replaces ABMultiply (1D-row-block/mulmat_kernel.cu:86-92,
+ * 2D-block-matrix/mulmat_kernel.cu:81-87):  C[m x n] = A[m x k] . B[k x n].
+ * `w` is the reference's tile width argument; it must be positive and is
+ * otherwise ignored (the tile is chosen by the library).
+ * Asynchronous on `stream`. */
+int mgx_dgemm(const double *A, const double *B, double *C,
+              int64_t m, int64_t n, int64_t k,
+              int64_t lda, int64_t ldb, int64_t ldc,
+              int w, void *stream);
+
+/* Replaces ABMultiplyAccumulateAsync (1D-row-block/mulmat_kernel.cu:148-158,
+ * 2D-block-matrix/mulmat_kernel.cu:142-151):
+ *   C += A[:, k_offset : k_offset + k_chunk] . B_chunk
+ * B_chunk is a packed k_chunk x n block (ldb).  Columns of A with
+ * k_offset + kk >= k_total contribute nothing (the reference loads them as
+ * zero).  The chunk partial product is formed first and then added to C
+ * once, as in the reference.  Asynchronous on `stream`. */
+int mgx_dgemm_accumulate(const double *A, const double *B_chunk, double *C,
+                         int64_t m, int64_t n, int64_t k_total,
+                         int64_t k_offset, int64_t k_chunk,
+                         int64_t lda, int64_t ldb, int64_t ldc,
+                         int w, void *stream);
+
+/* ---- constant-result validation ----------------------------------------
+ * Replaces validate_matrix_C (1D-row-block/mulmat_kernel.cu:228-326).
+ * An element is invalid iff |C-expected| > abs_tol AND |C-expected|/|expected|
+ * > rel_tol (the relative error is 0 when expected == 0), exactly the
+ * reference rule (1D-row-block/mulmat_kernel.cu:180-187).  Synchronous.
+ * Prints the "NUMERICAL VALIDATION" block to stdout when rank == 0.
+ * Returns MGX_OK on a completed check; *pass_out = 1 if no element is
+ * invalid.  Any of the out pointers may be NULL. */
+int mgx_validate_constant(const double *dC, int64_t rows, int64_t cols,
+                          double expected, double abs_tol, double rel_tol,
+                          int rank, int *pass_out, uint64_t *invalid_out,
+                          double *max_abs_out, double *max_rel_out);
+
+/* ---- split-precision (Ozaki) GEMM ---------------------------------------
+ * C = A.B (accumulate == 0) or C += A.B (accumulate != 0), FP64 in and out,
+ * computed from `slices` INT8 (or BF16) slices of the power-of-two scaled
+ * rows of A and columns of B.  Slice-pair products run on the INT8 (BF16)
+ * MFMA cores with exact INT32 (FP32) accumulation; FP64 is used for the
+ * exponent scan, the slice extraction and the reconstruction into C.
+ * No counterpart in the reference: it is a second implementation of the
+ * same local product (ABMultiply / ABMultiplyAccumulateAsync semantics).
+ *
+ * Workspace: `workspace` (device, 256-byte aligned) of `workspace_bytes`
+ * >= mgx_split_workspace_bytes(...); pass workspace == NULL to use a
+ * library-owned per-device buffer grown on demand.
+ * Inputs must be finite; a row of A or column of B holding Inf/NaN
+ * produces NaN in the corresponding row/column of C. */
+size_t mgx_split_workspace_bytes(int form, int slices,
+                                 int64_t m, int64_t n, int64_t k);
+int mgx_split_gemm(int form, int slices,
+                   const double *A, const double *B, double *C,
+                   int64_t m, int64_t n, int64_t k,
+                   int64_t lda, int64_t ldb, int64_t ldc,
+                   int accumulate, void *workspace, size_t workspace_bytes,
+                   void *stream);
+
+/* mgx_split_gemm that also records four caller-created hipEvent_t handles
+ * (passed as void*; NULL entries are skipped) on `stream` at the phase
+ * boundaries: [0] before the exponent scan, [1] before slice extraction,
+ * [2] before the slice-pair product kernel, [3] after it.  Asynchronous;
+ * the caller reads hipEventElapsedTime afterwards.  Used to time the
+ * product kernel inside a timed region without synchronising. */
+int mgx_split_gemm_events(int form, int slices,
+                          const double *A, const double *B, double *C,
+                          int64_t m, int64_t n, int64_t k,
+                          int64_t lda, int64_t ldb, int64_t ldc,
+                          int accumulate, void *workspace, size_t workspace_bytes,
+                          void *stream, void *const *events);
+
+/* Supported slice-count range for a split form (inclusive). */
+int mgx_split_slices_range(int form, int *min_slices, int *max_slices);
+
+/* ---- synthetic inputs ---------------------------------------------------
+ * Fill a rows x cols row-major device matrix (ld) on `stream`.
+ * kind 0: constant `value` (the reference init: A = 1, B = 2,
+ *         1D-row-block/mmb.cu:571-605, 2D-block-matrix/mmb.cu:151-154).
+ * kind 1: uniform[-1,1): element (r,c) = 2*(x>>11)*2^-53 - 1 with
+ *         x = the (r*cols+c+1)-th output of splitmix64 seeded with `seed`.
+ * kind 2: accuracy stress: element i = r*cols+c takes x = output 2i+1 and
+ *         y = output 2i+2 of the same generator; value =
+ *         (y&1 ? -1 : 1) * (1 + (x>>12)*2^-52) * 2^((y>>11) % 61 - 30).
+ *         The row/column scaling of the stress inputs is applied separately
+ *         (mgx_fill_exponents + mgx_scale_rows_cols). */
+int mgx_fill(double *X, int64_t rows, int64_t cols, int64_t ld,
+             int kind, uint64_t seed, double value, void *stream);
+
+/* e[i] = lo + ((output i+1 of splitmix64(seed)) >> 11) % (hi-lo+1);
+ * e is a device int32 array. */
+int mgx_fill_exponents(int32_t *e, int64_t count, uint64_t seed, int lo, int hi,
+                       void *stream);
+
+/* X[r][c] *= 2^(row_exp[r] + col_exp[c]); either exponent array may be NULL.
+ * Exponent arrays are device int32 arrays. */
+int mgx_scale_rows_cols(double *X, int64_t rows, int64_t cols, int64_t ld,
+                        const int32_t *row_exp, const int32_t *col_exp,
+                        void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MGX_H */

@@ -1,0 +1,113 @@
+"""mgxbench_amd: the MGXBench local FP64 block product on MI355X (gfx950).
+
+Python face of libmgx.  The operator functions keep the names, argument
+order and meaning of the reference's operator API
+(1D-row-block/mulmat_kernel.cu:86, 148, 228):
+
+    ABMultiply(a, b, c, m, n, k, lda, ldb, ldc, w)
+    ABMultiplyAccumulateAsync(a, b_chunk, c, m, n, k_total, k_offset, k_chunk,
+                              lda, ldb, ldc, w, stream)
+    validate_matrix_C(d_C, rows, cols, expected, abs_tol, rel_tol, rank) -> int
+
+Matrix arguments are device pointers given as ints or as torch tensors on a
+HIP device; `stream` is a torch.cuda.Stream, a raw hipStream_t int, or None
+(torch's current stream).  Unlike the reference, a failing call raises
+MgxError instead of aborting the process.
+"""
+from __future__ import annotations
+
+from . import _lib
+from ._lib import FORM_BF16, FORM_FP64, FORM_INT8, MgxError, lib
+
+__all__ = [
+    "ABMultiply", "ABMultiplyAccumulateAsync", "validate_matrix_C", "split_gemm",
+    "split_gemm_events", "split_workspace_bytes", "fill", "fill_exponents", "scale_rows_cols", "version", "peak",
+    "FORM_FP64", "FORM_INT8", "FORM_BF16", "MgxError", "lib",
+]
+
+
+def _ptr(x) -> int:
+    if x is None:
+        return 0
+    if isinstance(x, int):
+        return x
+    return int(x.data_ptr())
+
+
+def _stream(stream) -> int:
+    if stream is None:
+        import torch
+        return int(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)
+
+
+def version() -> str:
+    return lib().mgx_version().decode()
+
+
+def peak(form: int) -> float:
+    return float(lib().mgx_peak(form))
+
+
+def ABMultiply(a, b, c, m, n, k, lda, ldb, ldc, w=32, stream=None) -> None:
+    """C = A.B (overwrite), asynchronous on `stream` (reference: default stream)."""
+    _lib.call("mgx_dgemm", _ptr(a), _ptr(b), _ptr(c), m, n, k, lda, ldb, ldc, w, _stream(stream))
+
+
+def ABMultiplyAccumulateAsync(a, b_chunk, c, m, n, k_total, k_offset, k_chunk, lda, ldb, ldc,
+                              w=32, stream=None) -> None:
+    """C += A[:, k_offset:k_offset+k_chunk] . B_chunk, asynchronous on `stream`."""
+    _lib.call("mgx_dgemm_accumulate", _ptr(a), _ptr(b_chunk), _ptr(c), m, n, k_total, k_offset,
+              k_chunk, lda, ldb, ldc, w, _stream(stream))
+
+
+def validate_matrix_C(d_C, rows, cols, expected, abs_tol, rel_tol, rank=0, details=False):
+    """Reference rule; returns 1 (pass) or 0 (fail), or a dict when details=True."""
+    import ctypes
+    ok = ctypes.c_int(0)
+    bad = ctypes.c_uint64(0)
+    ma = ctypes.c_double(0.0)
+    mr = ctypes.c_double(0.0)
+    _lib.call("mgx_validate_constant", _ptr(d_C), rows, cols, expected, abs_tol, rel_tol, rank,
+              ctypes.byref(ok), ctypes.byref(bad), ctypes.byref(ma), ctypes.byref(mr))
+    if details:
+        return {"pass": ok.value, "invalid": bad.value, "max_abs": ma.value, "max_rel": mr.value}
+    return ok.value
+
+
+def split_workspace_bytes(form, slices, m, n, k) -> int:
+    return int(lib().mgx_split_workspace_bytes(form, slices, m, n, k))
+
+
+def split_gemm(a, b, c, m, n, k, lda, ldb, ldc, *, form=FORM_INT8, slices=8, accumulate=False,
+               workspace=None, workspace_bytes=0, stream=None) -> None:
+    """FP64-accurate C = A.B (or C += A.B) through INT8 slice products."""
+    _lib.call("mgx_split_gemm", form, slices, _ptr(a), _ptr(b), _ptr(c), m, n, k, lda, ldb, ldc,
+              1 if accumulate else 0, _ptr(workspace), workspace_bytes, _stream(stream))
+
+
+def split_gemm_events(a, b, c, m, n, k, lda, ldb, ldc, events, *, form=FORM_INT8, slices=8,
+                      accumulate=False, workspace=None, workspace_bytes=0, stream=None) -> None:
+    """split_gemm recording 4 torch.cuda.Event objects at its phase boundaries
+    (before scan, before slicing, before the product kernel, after it)."""
+    import ctypes
+    handles = (ctypes.c_void_p * 4)(*[int(e.cuda_event) if e is not None else 0 for e in events])
+    _lib.call("mgx_split_gemm_events", form, slices, _ptr(a), _ptr(b), _ptr(c), m, n, k, lda,
+              ldb, ldc, 1 if accumulate else 0, _ptr(workspace), workspace_bytes,
+              _stream(stream), handles)
+
+
+def fill(x, rows, cols, ld, kind, seed=0, value=0.0, stream=None) -> None:
+    """kind 0: constant, 1: uniform[-1,1), 2: accuracy-stress values."""
+    _lib.call("mgx_fill", _ptr(x), rows, cols, ld, kind, seed, value, _stream(stream))
+
+
+def fill_exponents(e, count, seed, lo, hi, stream=None) -> None:
+    _lib.call("mgx_fill_exponents", _ptr(e), count, seed, lo, hi, _stream(stream))
+
+
+def scale_rows_cols(x, rows, cols, ld, row_exp=None, col_exp=None, stream=None) -> None:
+    _lib.call("mgx_scale_rows_cols", _ptr(x), rows, cols, ld, _ptr(row_exp), _ptr(col_exp),
+              _stream(stream))

@@ -1,0 +1,531 @@
+// FP64-accurate split-precision GEMM (Ozaki scheme I) on the INT8 matrix
+// cores of gfx950.
+//
+// No counterpart in the reference: it computes the same local product as
+// ABMultiply / ABMultiplyAccumulateAsync (1D-row-block/mulmat_kernel.cu:86,
+// 148) through a different arithmetic.
+//
+// Algorithm (INT8 form, BITS = 7 bits per slice):
+//  1. e_i = frexp exponent of max_k |A_ik| (so |A_ik| < 2^e_i), f_j likewise
+//     for column j of B.  A row/column holding Inf/NaN gets a sentinel and
+//     its outputs become NaN.
+//  2. x = A_ik * 2^-e_i in (-1,1).  Slice p (1..s) is
+//        sigma_p = trunc(x * 2^(7p)) - 128 * trunc(x * 2^(7(p-1)))
+//     an integer in [-127,127]; sum_p sigma_p 2^(-7p) is x truncated to 7s
+//     fractional bits.  Every step is exact in FP64.  B is sliced per column
+//     with f_j and stored transposed (n-major, k contiguous).
+//  3. For each weight group t = 2..s+1 (slice pairs p+q = t), the INT8 MFMA
+//     accumulates G_t = sum_{p+q=t} sum_k sigmaA_p sigmaB_q exactly in INT32.
+//     The K range is cut into chunks of `chunk` 64-deep tiles, chosen so
+//     that pairs(t) * chunk * 64 * 127^2 < 2^31; after each chunk the lane
+//     folds v = fma((double)G, 2^-7t, v) in FP64 (exact product, one rounding).
+//  4. C_ij = ldexp(v, e_i + f_j)  (or C_ij + that for the accumulate form).
+// Pairs with p+q > s+1 are dropped; their weight is below 2^(-7s) relative.
+#include "mgx_internal.hpp"
+
+#include <algorithm>
+
+namespace {
+
+constexpr int KT = 64;             // k depth of one INT8 MFMA step / LDS stage
+constexpr int K_ALIGN = 256;       // slice rows are zero-padded to this multiple
+constexpr int EXP_BAD = 0x40000000;
+constexpr int MIN_SLICES = 1;
+constexpr int MAX_SLICES = 9;
+constexpr long long SLICE_MAX_PRODUCT = 127ll * 127ll;
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+inline int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+// ---- exponent scan --------------------------------------------------------
+
+__global__ void zero_u64_kernel(unsigned long long *p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+__device__ inline unsigned long long abs_bits(double x) {
+    return (unsigned long long)__double_as_longlong(x) & 0x7fffffffffffffffull;
+}
+
+// Row maxima of A (m x k, lda): grid.x covers k in segments of 256*16,
+// grid.y strides over rows.  Maxima are kept as the bit patterns of |x|,
+// which order like the values for non-negative doubles (NaN above Inf).
+__global__ __launch_bounds__(256) void row_absmax_kernel(const double *__restrict__ A, int64_t m,
+                                                         int64_t k, int64_t lda,
+                                                         unsigned long long *rowmax) {
+    __shared__ unsigned long long sh[4];
+    const int64_t kbase = (int64_t)blockIdx.x * (256 * 16);
+    for (int64_t r = blockIdx.y; r < m; r += gridDim.y) {
+        unsigned long long mx = 0;
+        const double *row = A + r * lda;
+#pragma unroll 4
+        for (int i = 0; i < 16; ++i) {
+            const int64_t kk = kbase + i * 256 + threadIdx.x;
+            if (kk < k) mx = max(mx, abs_bits(row[kk]));
+        }
+        for (int off = 32; off > 0; off >>= 1) mx = max(mx, (unsigned long long)__shfl_xor(mx, off));
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            mx = max(max(sh[0], sh[1]), max(sh[2], sh[3]));
+            if (mx) atomicMax(rowmax + r, mx);
+        }
+        __syncthreads();
+    }
+}
+
+// Column maxima of B (k x n, ldb): one lane per column, grid.y splits k in
+// segments of 256 rows.
+__global__ __launch_bounds__(256) void col_absmax_kernel(const double *__restrict__ B, int64_t k,
+                                                         int64_t n, int64_t ldb,
+                                                         unsigned long long *colmax) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const int64_t k0 = (int64_t)blockIdx.y * 256;
+    const int64_t k1 = min(k, k0 + 256);
+    unsigned long long mx = 0;
+    for (int64_t kk = k0; kk < k1; ++kk) mx = max(mx, abs_bits(B[kk * ldb + c]));
+    if (mx) atomicMax(colmax + c, mx);
+}
+
+__device__ inline int exponent_of_bits(unsigned long long bits) {
+    if (bits == 0) return 0;
+    if (bits >= 0x7ff0000000000000ull) return EXP_BAD;
+    int e;
+    frexp(__longlong_as_double((long long)bits), &e);
+    return e;
+}
+
+__global__ void exponents_kernel(const unsigned long long *maxbits, int32_t *e, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        e[i] = exponent_of_bits(maxbits[i]);
+}
+
+// ---- slice extraction -----------------------------------------------------
+
+// Writes the s slices of 16 consecutive scaled values x[0..15] as 16 bytes
+// each.  dst points at slice 1; consecutive slices are `slice_stride` apart.
+__device__ inline void emit_slices16(const double *x, int s, int8_t *dst, int64_t slice_stride) {
+    double prev[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) prev[i] = 0.0;
+    double scale = 1.0;
+    for (int p = 1; p <= s; ++p) {
+        scale *= 128.0;
+        v4i packed;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            unsigned int word = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int i = w * 4 + b;
+                const double tp = trunc(x[i] * scale);
+                const int sig = (int)(tp - 128.0 * prev[i]);
+                prev[i] = tp;
+                word |= ((unsigned int)(sig & 0xff)) << (8 * b);
+            }
+            packed[w] = (int)word;
+        }
+        *(v4i *)(dst + (int64_t)(p - 1) * slice_stride) = packed;
+    }
+}
+
+// A slices: Asl[p][row][kp], kp < Kp (zero beyond k).  One lane per 16
+// consecutive k of one row.
+__global__ __launch_bounds__(256) void slice_rows_kernel(const double *__restrict__ A, int64_t m,
+                                                         int64_t k, int64_t lda,
+                                                         const int32_t *__restrict__ rexp, int s,
+                                                         int64_t Kp, int8_t *__restrict__ Asl) {
+    const int64_t groups_per_row = Kp / 16;
+    const int64_t total = m * groups_per_row;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = g / groups_per_row;
+        const int64_t k0 = (g - r * groups_per_row) * 16;
+        const int e = rexp[r];
+        const double *row = A + r * lda;
+        double x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int64_t kk = k0 + i;
+            x[i] = (kk < k && e != EXP_BAD) ? ldexp(row[kk], -e) : 0.0;
+        }
+        emit_slices16(x, s, Asl + r * Kp + k0, m * Kp);
+    }
+}
+
+// B slices, transposed: Bsl[q][col][kp].  A workgroup covers 64 k x 64
+// columns; lane t takes column t/4 and the 16 k values (t%4)*16.. of it, so
+// four lanes write one contiguous 64-byte run of a slice row.
+__global__ __launch_bounds__(256) void slice_cols_kernel(const double *__restrict__ B, int64_t k,
+                                                         int64_t n, int64_t ldb,
+                                                         const int32_t *__restrict__ cexp, int s,
+                                                         int64_t Kp, int8_t *__restrict__ Bsl) {
+    const int64_t c = (int64_t)blockIdx.y * 64 + (threadIdx.x >> 2);
+    const int64_t k0 = (int64_t)blockIdx.x * 64 + (threadIdx.x & 3) * 16;
+    if (c >= n) return;
+    const int e = cexp[c];
+    double x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int64_t kk = k0 + i;
+        x[i] = (kk < k && e != EXP_BAD) ? ldexp(B[kk * ldb + c], -e) : 0.0;
+    }
+    emit_slices16(x, s, Bsl + c * Kp + k0, n * Kp);
+}
+
+// ---- fused slice-pair GEMM --------------------------------------------------
+
+constexpr int OBM = 128;
+constexpr int OBN = 128;
+constexpr int OSTR = KT + 16;  // LDS row stride in bytes
+constexpr int ONT = 256;
+constexpr int OGROUP_M = 8;
+
+// Walks the (group t, K chunk, pair p, k tile) steps in execution order.
+struct StepIter {
+    int s, ktiles;
+    int t, pmin, pmax, p, c0, c1, kt, chunk;
+
+    __device__ void start(int s_, int ktiles_) {
+        s = s_;
+        ktiles = ktiles_;
+        t = 2;
+        begin_group();
+    }
+    __device__ void begin_group() {
+        pmin = max(1, t - s);
+        pmax = min(s, t - 1);
+        const long long np = pmax - pmin + 1;
+        chunk = (int)max(1ll, 2147483647ll / (np * SLICE_MAX_PRODUCT * KT));
+        c0 = 0;
+        c1 = min(chunk, ktiles);
+        p = pmin;
+        kt = 0;
+    }
+    // Moves past the current step.  Returns true when that step closed an
+    // accumulation chunk of the group that was current before the call.
+    __device__ bool advance() {
+        if (++kt < c1) return false;
+        if (++p <= pmax) {
+            kt = c0;
+            return false;
+        }
+        c0 = c1;
+        if (c0 < ktiles) {
+            c1 = min(c0 + chunk, ktiles);
+            p = pmin;
+            kt = c0;
+            return true;
+        }
+        ++t;
+        if (t <= s + 1) begin_group();
+        return true;
+    }
+};
+
+__global__ __launch_bounds__(ONT, 2) void ozaki_int8_gemm_128x128(
+    const int8_t *__restrict__ Asl, const int8_t *__restrict__ Bsl,
+    const int32_t *__restrict__ rexp, const int32_t *__restrict__ cexp, double *__restrict__ C,
+    int64_t M, int64_t N, int64_t Kp, int64_t ldc, int s, int64_t nsteps, int accumulate) {
+    __shared__ __attribute__((aligned(16))) int8_t As[2][OBM * OSTR];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[2][OBN * OSTR];
+
+    const int tiles_m = (int)((M + OBM - 1) / OBM);
+    const int tiles_n = (int)((N + OBN - 1) / OBN);
+    int tm, tn;
+    {
+        const int nb = tiles_m * tiles_n;
+        int b = blockIdx.x;
+        if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+        const int per_group = OGROUP_M * tiles_n;
+        const int g = b / per_group;
+        const int first_m = g * OGROUP_M;
+        const int gsize = min(tiles_m - first_m, OGROUP_M);
+        const int r = b - g * per_group;
+        tm = first_m + r % gsize;
+        tn = r / gsize;
+    }
+    const int64_t m0 = (int64_t)tm * OBM;
+    const int64_t n0 = (int64_t)tn * OBN;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave >> 1;
+    const int wn = wave & 1;
+    const int lk = lane >> 4;
+    const int lc = lane & 15;
+
+    // Load assignment: lane takes 32 bytes (two 16-byte vectors) of one slab row.
+    const int ld_row = tid >> 1;
+    const int ld_off = (tid & 1) * 32;
+    const bool a_ok = m0 + ld_row < M;
+    const bool b_ok = n0 + ld_row < N;
+    const int64_t a_row_off = (a_ok ? m0 + ld_row : 0) * Kp + ld_off;
+    const int64_t b_row_off = (b_ok ? n0 + ld_row : 0) * Kp + ld_off;
+    const int64_t a_slice_stride = M * Kp;
+    const int64_t b_slice_stride = N * Kp;
+
+    v4i ra0, ra1, rb0, rb1;
+    const v4i vzero = {0, 0, 0, 0};
+    auto load_step = [&](int p, int q, int kt) {
+        const int8_t *pa = Asl + (int64_t)(p - 1) * a_slice_stride + a_row_off + (int64_t)kt * KT;
+        const int8_t *pb = Bsl + (int64_t)(q - 1) * b_slice_stride + b_row_off + (int64_t)kt * KT;
+        ra0 = a_ok ? *(const v4i *)pa : vzero;
+        ra1 = a_ok ? *(const v4i *)(pa + 16) : vzero;
+        rb0 = b_ok ? *(const v4i *)pb : vzero;
+        rb1 = b_ok ? *(const v4i *)(pb + 16) : vzero;
+    };
+    auto store_step = [&](int buf) {
+        *(v4i *)&As[buf][ld_row * OSTR + ld_off] = ra0;
+        *(v4i *)&As[buf][ld_row * OSTR + ld_off + 16] = ra1;
+        *(v4i *)&Bs[buf][ld_row * OSTR + ld_off] = rb0;
+        *(v4i *)&Bs[buf][ld_row * OSTR + ld_off + 16] = rb1;
+    };
+
+    v4i acc[4][4];
+    double accd[4][4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            acc[i][j] = vzero;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) accd[i][j][r] = 0.0;
+        }
+
+    const int ktiles = (int)(Kp / KT);
+    StepIter it;
+    it.start(s, ktiles);
+    load_step(it.p, it.t - it.p, it.kt);
+    store_step(0);
+    __syncthreads();
+
+    for (int64_t step = 0; step < nsteps; ++step) {
+        const int buf = (int)(step & 1);
+        const int t_cur = it.t;
+        const bool flush = it.advance();
+        const bool more = step + 1 < nsteps;
+        if (more) load_step(it.p, it.t - it.p, it.kt);
+
+        v4i af[4], bf[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            af[i] = *(const v4i *)&As[buf][(wm * 64 + i * 16 + lc) * OSTR + 16 * lk];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            bf[j] = *(const v4i *)&Bs[buf][(wn * 64 + j * 16 + lc) * OSTR + 16 * lk];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+
+        if (flush) {
+            const double w = ldexp(1.0, -7 * t_cur);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        accd[i][j][r] = fma((double)acc[i][j][r], w, accd[i][j][r]);
+                    acc[i][j] = vzero;
+                }
+        }
+        if (more) store_step(buf ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t row = m0 + wm * 64 + i * 16 + 4 * lk + r;
+            if (row >= M) continue;
+            const int er = rexp[row];
+            double *c_row = C + row * ldc;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int64_t col = n0 + wn * 64 + j * 16 + lc;
+                if (col >= N) continue;
+                const int ec = cexp[col];
+                double v;
+                if (er == EXP_BAD || ec == EXP_BAD)
+                    v = __longlong_as_double(0x7ff8000000000000ll);
+                else
+                    v = ldexp(accd[i][j][r], er + ec);
+                c_row[col] = accumulate ? c_row[col] + v : v;
+            }
+        }
+    }
+}
+
+__global__ void zero_matrix_kernel(double *C, int64_t m, int64_t n, int64_t ldc) {
+    const int64_t total = m * n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i / n;
+        C[r * ldc + (i - r * n)] = 0.0;
+    }
+}
+
+unsigned grid1d(int64_t total, int threads, int64_t cap = 65536) {
+    int64_t b = (total + threads - 1) / threads;
+    if (b > cap) b = cap;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+struct SplitLayout {
+    int64_t Kp;
+    size_t a_off, b_off, rmax_off, cmax_off, rexp_off, cexp_off, total;
+};
+
+SplitLayout split_layout(int s, int64_t m, int64_t n, int64_t k) {
+    SplitLayout L;
+    L.Kp = round_up(k < 1 ? 1 : k, K_ALIGN);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    L.a_off = take((size_t)s * m * L.Kp);
+    L.b_off = take((size_t)s * n * L.Kp);
+    L.rmax_off = take((size_t)(m + n) * 8);
+    L.cmax_off = L.rmax_off + (size_t)m * 8;
+    L.rexp_off = take((size_t)m * 4);
+    L.cexp_off = take((size_t)n * 4);
+    L.total = off;
+    return L;
+}
+
+}  // namespace
+
+extern "C" int mgx_split_slices_range(int form, int *min_slices, int *max_slices) {
+    mgx::clear_error();
+    if (form != MGX_FORM_INT8)
+        return mgx::set_error(MGX_ERR_UNSUPPORTED, "split form not supported");
+    if (min_slices) *min_slices = MIN_SLICES;
+    if (max_slices) *max_slices = MAX_SLICES;
+    return MGX_OK;
+}
+
+extern "C" size_t mgx_split_workspace_bytes(int form, int slices, int64_t m, int64_t n,
+                                            int64_t k) {
+    if (form != MGX_FORM_INT8 || slices < MIN_SLICES || slices > MAX_SLICES || m < 0 || n < 0 ||
+        k < 0)
+        return 0;
+    return split_layout(slices, m, n, k).total;
+}
+
+static int split_gemm_impl(int form, int slices, const double *A, const double *B, double *C,
+                           int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc,
+                           int accumulate, void *workspace, size_t workspace_bytes, void *stream,
+                           void *const *events) {
+    mgx::clear_error();
+    if (form != MGX_FORM_INT8)
+        return mgx::set_error(MGX_ERR_UNSUPPORTED, "split form not supported");
+    if (slices < MIN_SLICES || slices > MAX_SLICES)
+        return mgx::set_error(MGX_ERR_UNSUPPORTED, "slice count out of range");
+    MGX_ARG_CHECK(m >= 0 && n >= 0 && k >= 0, "negative matrix dimension");
+    if (m == 0 || n == 0) return MGX_OK;
+    MGX_ARG_CHECK(C != nullptr && ldc >= n, "C is NULL or ldc < n");
+    hipStream_t st = (hipStream_t)stream;
+    if (k == 0) {
+        if (accumulate) return MGX_OK;
+        hipLaunchKernelGGL(zero_matrix_kernel, dim3(grid1d(m * n, 256)), dim3(256), 0, st, C, m,
+                           n, ldc);
+        return mgx::launch_status("zero_matrix_kernel");
+    }
+    MGX_ARG_CHECK(A != nullptr && B != nullptr, "A or B is NULL");
+    MGX_ARG_CHECK(lda >= k && ldb >= n, "lda < k or ldb < n");
+    const SplitLayout L = split_layout(slices, m, n, k);
+    const int64_t tiles = ((m + OBM - 1) / OBM) * ((n + OBN - 1) / OBN);
+    MGX_ARG_CHECK(tiles <= 0x7fffffffll, "matrix too large for one launch");
+
+    char *ws = (char *)workspace;
+    if (ws) {
+        if (workspace_bytes < L.total)
+            return mgx::set_error(MGX_ERR_WORKSPACE, "split workspace too small");
+        if (((uintptr_t)ws & 255) != 0)
+            return mgx::set_error(MGX_ERR_INVALID_ARG, "workspace must be 256-byte aligned");
+    } else {
+        int status = MGX_OK;
+        ws = (char *)mgx::device_scratch(mgx::SCRATCH_SPLIT, L.total, &status);
+        if (!ws) return status;
+    }
+    int8_t *Asl = (int8_t *)(ws + L.a_off);
+    int8_t *Bsl = (int8_t *)(ws + L.b_off);
+    unsigned long long *rmax = (unsigned long long *)(ws + L.rmax_off);
+    unsigned long long *cmax = (unsigned long long *)(ws + L.cmax_off);
+    int32_t *rexp = (int32_t *)(ws + L.rexp_off);
+    int32_t *cexp = (int32_t *)(ws + L.cexp_off);
+    int rc;
+    auto mark = [&](int i) -> int {
+        if (events && events[i]) MGX_HIP_TRY(hipEventRecord((hipEvent_t)events[i], st));
+        return MGX_OK;
+    };
+
+    if ((rc = mark(0)) != MGX_OK) return rc;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1d(m + n, 256)), dim3(256), 0, st, rmax, m + n);
+    if ((rc = mgx::launch_status("zero_u64_kernel")) != MGX_OK) return rc;
+    {
+        const unsigned gx = (unsigned)((k + 4095) / 4096);
+        const unsigned gy = (unsigned)std::min<int64_t>(m, 65535);
+        hipLaunchKernelGGL(row_absmax_kernel, dim3(gx, gy), dim3(256), 0, st, A, m, k, lda, rmax);
+        if ((rc = mgx::launch_status("row_absmax_kernel")) != MGX_OK) return rc;
+    }
+    {
+        const unsigned gx = (unsigned)((n + 255) / 256);
+        const int64_t gy = (k + 255) / 256;
+        MGX_ARG_CHECK(gy <= 65535, "k too large for the column scan");
+        hipLaunchKernelGGL(col_absmax_kernel, dim3(gx, (unsigned)gy), dim3(256), 0, st, B, k, n,
+                           ldb, cmax);
+        if ((rc = mgx::launch_status("col_absmax_kernel")) != MGX_OK) return rc;
+    }
+    hipLaunchKernelGGL(exponents_kernel, dim3(grid1d(m, 256)), dim3(256), 0, st, rmax, rexp, m);
+    if ((rc = mgx::launch_status("exponents_kernel")) != MGX_OK) return rc;
+    hipLaunchKernelGGL(exponents_kernel, dim3(grid1d(n, 256)), dim3(256), 0, st, cmax, cexp, n);
+    if ((rc = mgx::launch_status("exponents_kernel")) != MGX_OK) return rc;
+    if ((rc = mark(1)) != MGX_OK) return rc;
+
+    hipLaunchKernelGGL(slice_rows_kernel, dim3(grid1d(m * (L.Kp / 16), 256, 1 << 20)), dim3(256),
+                       0, st, A, m, k, lda, rexp, slices, L.Kp, Asl);
+    if ((rc = mgx::launch_status("slice_rows_kernel")) != MGX_OK) return rc;
+    {
+        const int64_t gy = (n + 63) / 64;
+        MGX_ARG_CHECK(gy <= 65535, "n too large for the column slicer");
+        hipLaunchKernelGGL(slice_cols_kernel, dim3((unsigned)(L.Kp / 64), (unsigned)gy), dim3(256),
+                           0, st, B, k, n, ldb, cexp, slices, L.Kp, Bsl);
+        if ((rc = mgx::launch_status("slice_cols_kernel")) != MGX_OK) return rc;
+    }
+    if ((rc = mark(2)) != MGX_OK) return rc;
+    const int64_t nsteps = (L.Kp / KT) * (int64_t)(slices * (slices + 1) / 2);
+    hipLaunchKernelGGL(ozaki_int8_gemm_128x128, dim3((unsigned)tiles), dim3(ONT), 0, st, Asl, Bsl,
+                       rexp, cexp, C, m, n, L.Kp, ldc, slices, nsteps, accumulate ? 1 : 0);
+    if ((rc = mgx::launch_status("ozaki_int8_gemm_128x128")) != MGX_OK) return rc;
+    return mark(3);
+}
+
+extern "C" int mgx_split_gemm(int form, int slices, const double *A, const double *B, double *C,
+                              int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb,
+                              int64_t ldc, int accumulate, void *workspace,
+                              size_t workspace_bytes, void *stream) {
+    return split_gemm_impl(form, slices, A, B, C, m, n, k, lda, ldb, ldc, accumulate, workspace,
+                           workspace_bytes, stream, nullptr);
+}
+
+extern "C" int mgx_split_gemm_events(int form, int slices, const double *A, const double *B,
+                                     double *C, int64_t m, int64_t n, int64_t k, int64_t lda,
+                                     int64_t ldb, int64_t ldc, int accumulate, void *workspace,
+                                     size_t workspace_bytes, void *stream, void *const *events) {
+    return split_gemm_impl(form, slices, A, B, C, m, n, k, lda, ldb, ldc, accumulate, workspace,
+                           workspace_bytes, stream, events);
+}

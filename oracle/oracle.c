@@ -1,0 +1,397 @@
+/*
+ * oracle.c -- CPU oracle for the local FP64 block product.
+ *
+ * TEST INFRASTRUCTURE, NOT PRODUCT CODE.  Only tests/, the package's smoke
+ * check and bench.py's CPU-baseline leg load this library, and only as the
+ * checker (or, for the baseline, as the timed CPU restatement of the
+ * reference's product).  The product path (libmgx) never calls it.
+ *
+ * Contents
+ *  - orc_splitmix64_at / orc_fill*: the synthetic-input definition shared
+ *    with libmgx's fill kernels (splitmix64 counter stream).
+ *  - orc_gemm_kernel_order: restatement of the reference GEMM arithmetic,
+ *    sharedABMultiply (1D-row-block/mulmat_kernel.cu:39-80): every C element
+ *    is one sequential chain of fused multiply-adds over k = 0..K-1 starting
+ *    from +0.0 (nvcc contracts `sum += a*b` into DFMA; out-of-range tile
+ *    lanes add 0*0, which leaves the chain unchanged), then stored.
+ *  - orc_gemm_accumulate_order: restatement of sharedABMultiplyAccumulate
+ *    (1D-row-block/mulmat_kernel.cu:100-142, 2D-block-matrix/mulmat_kernel.cu:
+ *    94-136): a chunk partial by the same chain over kk < k_chunk with
+ *    k_offset + kk < k_total, then C += partial once.
+ *  - orc_gemm_dd: double-double dot products (Ogita-Rump-Oishi Dot2 with
+ *    Dekker/Veltkamp TwoProduct realised by fma and Knuth TwoSum), giving an
+ *    accuracy reference far below FP64 rounding.
+ *  - orc_validate_constant: the reference validation rule
+ *    (1D-row-block/mulmat_kernel.cu:180-187) and the 2D host rule
+ *    (2D-block-matrix/mmb.cu:381-385).
+ *  - orc_split_*: a bit-exact model of libmgx's INT8 split-precision
+ *    (Ozaki scheme I) product: exponents, slices, exact group sums with the
+ *    same INT32 flush chunks, and the same FP64 reconstruction sequence.
+ *
+ * Parity pins (tests/test_oracle.py): exact rational brute force on tiny
+ * inputs for the fma chains, the accumulate form, the double-double dot and
+ * the split reconstruction; the A=1/B=2 known answer; integer-valued inputs
+ * (exact in every order); transposition identity; the splitmix64 published
+ * first outputs for seed 0; closed-form validation cases.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_API __attribute__((visibility("default")))
+
+static void set_threads(int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+}
+
+ORC_API int orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ---- synthetic inputs ---------------------------------------------------- */
+
+ORC_API uint64_t orc_splitmix64_at(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + i * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+ORC_API void orc_fill(double *X, int64_t rows, int64_t cols, int64_t ld, int kind, uint64_t seed,
+                      double value) {
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < rows; ++r) {
+        for (int64_t c = 0; c < cols; ++c) {
+            const uint64_t idx = (uint64_t)(r * cols + c);
+            double v;
+            if (kind == 0) {
+                v = value;
+            } else if (kind == 1) {
+                v = 2.0 * (double)(orc_splitmix64_at(seed, idx + 1) >> 11) * 0x1.0p-53 - 1.0;
+            } else {
+                const uint64_t x = orc_splitmix64_at(seed, 2 * idx + 1);
+                const uint64_t y = orc_splitmix64_at(seed, 2 * idx + 2);
+                const double mant = 1.0 + (double)(x >> 12) * 0x1.0p-52;
+                const int e = (int)((y >> 11) % 61ull) - 30;
+                v = ldexp((y & 1ull) ? -mant : mant, e);
+            }
+            X[r * ld + c] = v;
+        }
+    }
+}
+
+ORC_API void orc_fill_exponents(int32_t *e, int64_t count, uint64_t seed, int lo, int hi) {
+    const uint64_t span = (uint64_t)(hi - lo + 1);
+    for (int64_t i = 0; i < count; ++i)
+        e[i] = lo + (int)((orc_splitmix64_at(seed, (uint64_t)i + 1) >> 11) % span);
+}
+
+ORC_API void orc_scale_rows_cols(double *X, int64_t rows, int64_t cols, int64_t ld,
+                                 const int32_t *re, const int32_t *ce) {
+    for (int64_t r = 0; r < rows; ++r)
+        for (int64_t c = 0; c < cols; ++c)
+            X[r * ld + c] = ldexp(X[r * ld + c], (re ? re[r] : 0) + (ce ? ce[c] : 0));
+}
+
+/* ---- reference-order GEMM ------------------------------------------------ */
+
+/* Per-element chain: acc = fma(A[i][kk], B[kk][j], acc) for kk = 0..kd-1,
+ * starting from +0.0.  Rows are blocked (RB) and columns tiled (JB) so B is
+ * streamed once per row block; the per-element order is unchanged. */
+#define ORC_RB 16
+#define ORC_JB 512
+static void chain_block(const double *A, const double *B, double *P, int64_t m, int64_t n,
+                        int64_t kd, int64_t lda, int64_t ldb, int64_t ldp) {
+    const int64_t nrb = (m + ORC_RB - 1) / ORC_RB;
+    const int64_t njb = (n + ORC_JB - 1) / ORC_JB;
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+    for (int64_t rb = 0; rb < nrb; ++rb) {
+        for (int64_t jb = 0; jb < njb; ++jb) {
+            const int64_t i0 = rb * ORC_RB, i1 = i0 + ORC_RB < m ? i0 + ORC_RB : m;
+            const int64_t j0 = jb * ORC_JB, j1 = j0 + ORC_JB < n ? j0 + ORC_JB : n;
+            const int64_t w = j1 - j0;
+            double acc[ORC_RB][ORC_JB];
+            for (int64_t i = i0; i < i1; ++i)
+                for (int64_t j = 0; j < w; ++j) acc[i - i0][j] = 0.0;
+            for (int64_t kk = 0; kk < kd; ++kk) {
+                const double *brow = B + kk * ldb + j0;
+                for (int64_t i = i0; i < i1; ++i) {
+                    const double a = A[i * lda + kk];
+                    double *ar = acc[i - i0];
+                    for (int64_t j = 0; j < w; ++j) ar[j] = fma(a, brow[j], ar[j]);
+                }
+            }
+            for (int64_t i = i0; i < i1; ++i)
+                for (int64_t j = 0; j < w; ++j) P[i * ldp + j0 + j] = acc[i - i0][j];
+        }
+    }
+}
+
+ORC_API void orc_gemm_kernel_order(const double *A, const double *B, double *C, int64_t m,
+                                   int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc,
+                                   int nthreads) {
+    set_threads(nthreads);
+    chain_block(A, B, C, m, n, k, lda, ldb, ldc);
+}
+
+ORC_API int orc_gemm_accumulate_order(const double *A, const double *Bc, double *C, int64_t m,
+                                      int64_t n, int64_t k_total, int64_t k_offset,
+                                      int64_t k_chunk, int64_t lda, int64_t ldb, int64_t ldc,
+                                      int nthreads) {
+    set_threads(nthreads);
+    int64_t kd = k_total - k_offset;
+    if (kd > k_chunk) kd = k_chunk;
+    if (kd < 0) kd = 0;
+    double *P = (double *)malloc(sizeof(double) * (size_t)(m * n > 0 ? m * n : 1));
+    if (!P) return -1;
+    chain_block(A + k_offset, Bc, P, m, n, kd, lda, ldb, n);
+    for (int64_t i = 0; i < m; ++i)
+        for (int64_t j = 0; j < n; ++j) C[i * ldc + j] = C[i * ldc + j] + P[i * n + j];
+    free(P);
+    return 0;
+}
+
+/* ---- double-double reference --------------------------------------------- */
+
+/* hi + lo ~ sum_k A[i][k] B[k][j] with Dot2: TwoProduct(a,b) = (a*b, fma(a,b,-a*b)),
+ * TwoSum(x,y) = Knuth's branch-free six-operation sum.  Only the rows listed
+ * in `rows` (nrows entries) are computed; output row r of Chi/Clo holds
+ * source row rows[r]. */
+ORC_API void orc_gemm_dd_rows(const double *A, const double *B, double *Chi, double *Clo,
+                              const int64_t *rows, int64_t nrows, int64_t n, int64_t k,
+                              int64_t lda, int64_t ldb, int64_t ldc, int nthreads) {
+    set_threads(nthreads);
+    const int64_t njb = (n + ORC_JB - 1) / ORC_JB;
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+    for (int64_t r = 0; r < nrows; ++r) {
+        for (int64_t jb = 0; jb < njb; ++jb) {
+            const int64_t i = rows[r];
+            const int64_t j0 = jb * ORC_JB, j1 = j0 + ORC_JB < n ? j0 + ORC_JB : n;
+            const int64_t w = j1 - j0;
+            double p[ORC_JB], s[ORC_JB];
+            for (int64_t j = 0; j < w; ++j) p[j] = s[j] = 0.0;
+            for (int64_t kk = 0; kk < k; ++kk) {
+                const double a = A[i * lda + kk];
+                const double *brow = B + kk * ldb + j0;
+                for (int64_t j = 0; j < w; ++j) {
+                    const double h = a * brow[j];
+                    const double e = fma(a, brow[j], -h);
+                    const double x = p[j] + h;
+                    const double z = x - p[j];
+                    const double q = (p[j] - (x - z)) + (h - z);
+                    p[j] = x;
+                    s[j] = s[j] + (q + e);
+                }
+            }
+            for (int64_t j = 0; j < w; ++j) {
+                /* renormalise: hi = fl(p + s), lo = error of that sum */
+                const double x = p[j] + s[j];
+                const double z = x - p[j];
+                const double lo = (p[j] - (x - z)) + (s[j] - z);
+                Chi[r * ldc + j0 + j] = x;
+                Clo[r * ldc + j0 + j] = lo;
+            }
+        }
+    }
+}
+
+/* |A|.|B| for the rows listed (componentwise-error denominators). */
+ORC_API void orc_gemm_abs_rows(const double *A, const double *B, double *D, const int64_t *rows,
+                               int64_t nrows, int64_t n, int64_t k, int64_t lda, int64_t ldb,
+                               int64_t ldd, int nthreads) {
+    set_threads(nthreads);
+    const int64_t njb = (n + ORC_JB - 1) / ORC_JB;
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+    for (int64_t r = 0; r < nrows; ++r) {
+        for (int64_t jb = 0; jb < njb; ++jb) {
+            const int64_t i = rows[r];
+            const int64_t j0 = jb * ORC_JB, j1 = j0 + ORC_JB < n ? j0 + ORC_JB : n;
+            const int64_t w = j1 - j0;
+            double acc[ORC_JB];
+            for (int64_t j = 0; j < w; ++j) acc[j] = 0.0;
+            for (int64_t kk = 0; kk < k; ++kk) {
+                const double a = fabs(A[i * lda + kk]);
+                const double *brow = B + kk * ldb + j0;
+                for (int64_t j = 0; j < w; ++j) acc[j] = fma(a, fabs(brow[j]), acc[j]);
+            }
+            for (int64_t j = 0; j < w; ++j) D[r * ldd + j0 + j] = acc[j];
+        }
+    }
+}
+
+/* ---- validation rules ------------------------------------------------------ */
+
+/* 1D rule: invalid iff |C-e| > abs_tol && |C-e|/|e| > rel_tol (rel = 0 when
+ * e == 0).  Max errors ignore NaN (fmax). Returns the invalid count. */
+ORC_API uint64_t orc_validate_constant(const double *C, int64_t rows, int64_t cols, int64_t ld,
+                                       double expected, double abs_tol, double rel_tol,
+                                       double *max_abs, double *max_rel) {
+    uint64_t bad = 0;
+    double ma = 0.0, mr = 0.0;
+    const double ae = fabs(expected);
+    for (int64_t r = 0; r < rows; ++r)
+        for (int64_t c = 0; c < cols; ++c) {
+            const double abs_err = fabs(C[r * ld + c] - expected);
+            const double rel_err = ae > 0.0 ? abs_err / ae : 0.0;
+            if ((abs_err > abs_tol) && (rel_err > rel_tol)) ++bad;
+            ma = fmax(ma, abs_err);
+            mr = fmax(mr, rel_err);
+        }
+    if (max_abs) *max_abs = ma;
+    if (max_rel) *max_rel = mr;
+    return bad;
+}
+
+/* 2D host rule: valid iff every element is finite and |C-e| <= 1e-9 + 1e-12|e|.
+ * Returns the first failing linear index, or -1 when all pass. */
+ORC_API int64_t orc_validate_2d(const double *C, int64_t count, double expected) {
+    const double tol = 1e-9 + 1e-12 * fabs(expected);
+    for (int64_t i = 0; i < count; ++i)
+        if (!isfinite(C[i]) || fabs(C[i] - expected) > tol) return i;
+    return -1;
+}
+
+/* ---- split-precision model (INT8 slices, 7 bits each) --------------------- */
+
+#define ORC_EXP_BAD 0x40000000
+#define ORC_KT 64
+#define ORC_KALIGN 256
+
+static int exponent_of_max(double mx, int bad) {
+    if (bad) return ORC_EXP_BAD;
+    if (mx == 0.0) return 0;
+    int e;
+    frexp(mx, &e);
+    return e;
+}
+
+/* Row exponents of A (m x k) and column exponents of B (k x n). */
+ORC_API void orc_split_exponents(const double *A, const double *B, int64_t m, int64_t n,
+                                 int64_t k, int64_t lda, int64_t ldb, int32_t *rexp,
+                                 int32_t *cexp) {
+    for (int64_t i = 0; i < m; ++i) {
+        double mx = 0.0;
+        int bad = 0;
+        for (int64_t kk = 0; kk < k; ++kk) {
+            const double a = A[i * lda + kk];
+            if (!isfinite(a)) bad = 1;
+            else mx = fmax(mx, fabs(a));
+        }
+        rexp[i] = exponent_of_max(mx, bad);
+    }
+    for (int64_t j = 0; j < n; ++j) {
+        double mx = 0.0;
+        int bad = 0;
+        for (int64_t kk = 0; kk < k; ++kk) {
+            const double b = B[kk * ldb + j];
+            if (!isfinite(b)) bad = 1;
+            else mx = fmax(mx, fabs(b));
+        }
+        cexp[j] = exponent_of_max(mx, bad);
+    }
+}
+
+/* Slices of one scaled value x in (-1,1): sig[p-1], p = 1..s. */
+ORC_API void orc_split_value(double x, int s, int8_t *sig) {
+    double prev = 0.0, scale = 1.0;
+    for (int p = 1; p <= s; ++p) {
+        scale *= 128.0;
+        const double tp = trunc(x * scale);
+        sig[p - 1] = (int8_t)(int)(tp - 128.0 * prev);
+        prev = tp;
+    }
+}
+
+/* Group-t chunk length in 64-deep tiles, the INT32 bound of the product kernel. */
+ORC_API int orc_split_chunk_tiles(int s, int t) {
+    const int pmin = t - s > 1 ? t - s : 1;
+    const int pmax = t - 1 < s ? t - 1 : s;
+    const long long np = pmax - pmin + 1;
+    long long c = 2147483647ll / (np * 127ll * 127ll * ORC_KT);
+    return (int)(c < 1 ? 1 : c);
+}
+
+/* Full model: C = A.B (or C += A.B) exactly as the INT8 split kernel does. */
+ORC_API int orc_split_gemm_int8(int s, const double *A, const double *B, double *C, int64_t m,
+                                int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc,
+                                int accumulate, int nthreads) {
+    set_threads(nthreads);
+    if (s < 1 || s > 9) return -1;
+    if (m == 0 || n == 0) return 0;
+    if (k == 0) {
+        if (!accumulate)
+            for (int64_t i = 0; i < m; ++i)
+                for (int64_t j = 0; j < n; ++j) C[i * ldc + j] = 0.0;
+        return 0;
+    }
+    int32_t *rexp = (int32_t *)malloc(sizeof(int32_t) * m);
+    int32_t *cexp = (int32_t *)malloc(sizeof(int32_t) * n);
+    int8_t *As = (int8_t *)malloc((size_t)s * m * k);
+    int8_t *Bs = (int8_t *)malloc((size_t)s * n * k); /* transposed: [q][j][kk] */
+    if (!rexp || !cexp || !As || !Bs) {
+        free(rexp); free(cexp); free(As); free(Bs);
+        return -2;
+    }
+    orc_split_exponents(A, B, m, n, k, lda, ldb, rexp, cexp);
+    for (int64_t i = 0; i < m; ++i)
+        for (int64_t kk = 0; kk < k; ++kk) {
+            int8_t sig[9];
+            const double x = rexp[i] == ORC_EXP_BAD ? 0.0 : ldexp(A[i * lda + kk], -rexp[i]);
+            orc_split_value(x, s, sig);
+            for (int p = 0; p < s; ++p) As[((size_t)p * m + i) * k + kk] = sig[p];
+        }
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t kk = 0; kk < k; ++kk) {
+            int8_t sig[9];
+            const double x = cexp[j] == ORC_EXP_BAD ? 0.0 : ldexp(B[kk * ldb + j], -cexp[j]);
+            orc_split_value(x, s, sig);
+            for (int p = 0; p < s; ++p) Bs[((size_t)p * n + j) * k + kk] = sig[p];
+        }
+    const int64_t ktiles = ((k + ORC_KALIGN - 1) / ORC_KALIGN * ORC_KALIGN) / ORC_KT;
+    int overflow = 0;
+#pragma omp parallel for schedule(dynamic, 1) collapse(2) reduction(| : overflow)
+    for (int64_t i = 0; i < m; ++i) {
+        for (int64_t j = 0; j < n; ++j) {
+            double v = 0.0;
+            for (int t = 2; t <= s + 1; ++t) {
+                const int pmin = t - s > 1 ? t - s : 1;
+                const int pmax = t - 1 < s ? t - 1 : s;
+                const int64_t chunk = orc_split_chunk_tiles(s, t);
+                const double w = ldexp(1.0, -7 * t);
+                for (int64_t c0 = 0; c0 < ktiles; c0 += chunk) {
+                    int64_t k0 = c0 * ORC_KT;
+                    int64_t k1 = (c0 + chunk) * ORC_KT;
+                    if (k1 > k) k1 = k;
+                    long long g = 0;
+                    for (int p = pmin; p <= pmax; ++p) {
+                        const int8_t *ar = As + ((size_t)(p - 1) * m + i) * k;
+                        const int8_t *br = Bs + ((size_t)(t - p - 1) * n + j) * k;
+                        for (int64_t kk = k0; kk < k1; ++kk) g += (long long)ar[kk] * br[kk];
+                    }
+                    if (g > 2147483647ll || g < -2147483648ll) overflow = 1;
+                    v = fma((double)g, w, v);
+                }
+            }
+            double res;
+            if (rexp[i] == ORC_EXP_BAD || cexp[j] == ORC_EXP_BAD)
+                res = NAN;
+            else
+                res = ldexp(v, rexp[i] + cexp[j]);
+            C[i * ldc + j] = accumulate ? C[i * ldc + j] + res : res;
+        }
+    }
+    free(rexp); free(cexp); free(As); free(Bs);
+    return overflow ? -3 : 0;
+}
