@@ -127,6 +127,51 @@ int mgx_split_gemm_events(int form, int slices,
 /* Supported slice-count range for a split form (inclusive). */
 int mgx_split_slices_range(int form, int *min_slices, int *max_slices);
 
+/* ---- SUMMA panel streaming ----------------------------------------------
+ * Replaces the local part of the 2D SUMMA panel loop
+ * (2D-block-matrix/mmb.cu:239-371): for panels of width `panel` (ragged last
+ * panel), copy the m x kc column panel of the host A strip and the kc x n
+ * row panel of the host B strip to device double buffers on a transfer
+ * stream and run C += A_p . B_p on a compute stream, with event handoff
+ * (panel ready / compute done) in both directions.  The per-panel
+ * MPI/NCCL/NVSHMEM broadcasts of the reference are replaced by these
+ * host-to-device copies (the single-GPU stand-in).
+ * Host strips: pinned (mgx_host_alloc) strips are gathered directly by
+ * strided DMA; pageable strips are packed into library-owned pinned
+ * buffers first (the reference's pack_a / pack_b).  dC is a device m x n
+ * matrix that is accumulated into (zero it for C = A.B).  Synchronous: all
+ * resources are created before and released after the timed loop. */
+enum {
+    MGX_SUMMA_COPY_ASYNC = 0,      /* copies overlap compute (MMM, YYY, NNN codes) */
+    MGX_SUMMA_COPY_HOST_SYNC = 1   /* host waits for each panel copy (CCC, XXX, WWW, SSS) */
+};
+
+typedef struct mgx_summa_params {
+    int64_t m, n, k;    /* local C is m x n; reduction length k */
+    int64_t panel;      /* panel width (the reference's MGXBENCH_SUMMA_BK) */
+    int form;           /* MGX_FORM_FP64 or MGX_FORM_INT8 */
+    int slices;         /* slices of the INT8 form */
+    int copy_mode;      /* MGX_SUMMA_COPY_* */
+    int use_graph;      /* capture the whole panel loop in one hipGraph (pinned strips) */
+    int serialize;      /* debug: synchronise after every copy and every product */
+} mgx_summa_params;
+
+typedef struct mgx_summa_stats {
+    double seconds;     /* host wall time of the panel loop (launch to completion) */
+    int64_t panels;     /* number of panels, including a ragged last one */
+    int host_pinned;    /* 1 if the host strips were pinned (no host packing) */
+} mgx_summa_stats;
+
+int mgx_summa_run(const mgx_summa_params *params,
+                  const double *hA, int64_t lda,
+                  const double *hB, int64_t ldb,
+                  double *dC, int64_t ldc,
+                  mgx_summa_stats *stats);
+
+/* Pinned host memory for SUMMA strips (hipHostMalloc / hipHostFree). */
+int mgx_host_alloc(void **ptr, size_t bytes);
+int mgx_host_free(void *ptr);
+
 /* ---- synthetic inputs ---------------------------------------------------
  * Fill a rows x cols row-major device matrix (ld) on `stream`.
  * kind 0: constant `value` (the reference init: A = 1, B = 2,

@@ -23,7 +23,7 @@ BIN = PKG / "bin"
 LIB = PKG / "libmgx.so"
 INCLUDE = PKG.parent / "include"
 
-LIB_SOURCES = ["runtime.hip", "dgemm.hip", "ozaki.hip"]
+LIB_SOURCES = ["runtime.hip", "dgemm.hip", "ozaki.hip", "summa.hip"]
 EXE_SOURCES = {}  # name -> source file in csrc/, linked against libmgx.so
 
 ARCH = "gfx950"

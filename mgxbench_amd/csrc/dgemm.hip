@@ -140,13 +140,13 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma_128x128(
         __syncthreads();
     }
 
-    // 16x16 accumulator layout: lane l, register r holds row 4*(l/16)+r,
-    // column l%16 of the fragment.
+    // FP64 16x16 accumulator layout: lane l, register r holds row 4*r + l/16,
+    // column l%16 of the fragment (register r covers rows 4r..4r+3).
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int64_t row = m0 + wm * 64 + i * 16 + 4 * lk + r;
+            const int64_t row = m0 + wm * 64 + i * 16 + 4 * r + lk;
             if (row >= M) continue;
             double *c_row = C + row * ldc;
 #pragma unroll

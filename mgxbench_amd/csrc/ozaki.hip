@@ -16,10 +16,11 @@
 //     with f_j and stored transposed (n-major, k contiguous).
 //  3. For each weight group t = 2..s+1 (slice pairs p+q = t), the INT8 MFMA
 //     accumulates G_t = sum_{p+q=t} sum_k sigmaA_p sigmaB_q exactly in INT32.
-//     The K range is cut into chunks of `chunk` 64-deep tiles, chosen so
-//     that pairs(t) * chunk * 64 * 127^2 < 2^31; after each chunk the lane
-//     folds v = fma((double)G, 2^-7t, v) in FP64 (exact product, one rounding).
-//  4. C_ij = ldexp(v, e_i + f_j)  (or C_ij + that for the accumulate form).
+//     The K range is cut into chunks of `chunk` 64-deep tiles (whole LDS
+//     stages), chosen so that pairs(t) * chunk * 64 * 127^2 < 2^31.
+//  4. After each chunk the sums are folded into C in FP64:
+//     x = ldexp((double)G, e_i + f_j - 7t) (exact); the first fold writes
+//     C = x (or C_in + x for the accumulate form), later folds C = C + x.
 // Pairs with p+q > s+1 are dropped; their weight is below 2^(-7s) relative.
 #include "mgx_internal.hpp"
 
@@ -179,12 +180,28 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double *__restric
 }
 
 // ---- fused slice-pair GEMM --------------------------------------------------
+//
+// One workgroup of 256 lanes (4 wave64s, 2x2) owns a 256x256 block of C; each
+// wave owns 128x128 as 8x8 INT32 16x16 accumulators (256 registers per lane,
+// held in the accumulation registers).  A and B slice tiles of BK bytes of k
+// are staged through LDS with two buffers; the global loads of step s+1 are
+// issued before the MFMAs of step s.  Rows are padded by 16 bytes so the
+// 16-byte fragment reads of each 8-lane group hit distinct LDS banks.
+//
+// Steps run in (group t, K chunk, pair p, k tile) order.  At the end of every
+// K chunk the INT32 sums are folded into C in FP64:
+//     x = ldexp((double)G, e_i + f_j - 7t)      (exact)
+//     C = x  or  C_in + x   on the first fold (plain / accumulate form),
+//     C = C + x             on every later fold.
+// so C is read and written once per fold; C stays in HBM between folds.
 
-constexpr int OBM = 128;
-constexpr int OBN = 128;
-constexpr int OSTR = KT + 16;  // LDS row stride in bytes
-constexpr int ONT = 256;
-constexpr int OGROUP_M = 8;
+constexpr int TB = 256;         // C block edge handled by one workgroup
+constexpr int TBK = 128;        // k bytes per LDS stage
+constexpr int TSTR = TBK + 16;  // LDS row stride in bytes
+constexpr int TNT = 256;        // lanes per workgroup
+constexpr int TGROUP_M = 8;     // C block rows per raster group
+constexpr int FOLD_STR = 128 + 4;  // ints per row of a wave's fold scratch
+static_assert(4 * 32 * FOLD_STR * 4 <= 2 * TB * TSTR, "fold scratch must fit one LDS stage");
 
 // Walks the (group t, K chunk, pair p, k tile) steps in execution order.
 struct StepIter {
@@ -202,6 +219,7 @@ struct StepIter {
         pmax = min(s, t - 1);
         const long long np = pmax - pmin + 1;
         chunk = (int)max(1ll, 2147483647ll / (np * SLICE_MAX_PRODUCT * KT));
+        chunk = max(1, chunk / (TBK / KT)) * (TBK / KT);  // whole LDS stages
         c0 = 0;
         c1 = min(chunk, ktiles);
         p = pmin;
@@ -210,7 +228,8 @@ struct StepIter {
     // Moves past the current step.  Returns true when that step closed an
     // accumulation chunk of the group that was current before the call.
     __device__ bool advance() {
-        if (++kt < c1) return false;
+        kt += TBK / KT;
+        if (kt < c1) return false;
         if (++p <= pmax) {
             kt = c0;
             return false;
@@ -228,30 +247,30 @@ struct StepIter {
     }
 };
 
-__global__ __launch_bounds__(ONT, 2) void ozaki_int8_gemm_128x128(
+__global__ __launch_bounds__(TNT, 1) void ozaki_int8_gemm_256x256(
     const int8_t *__restrict__ Asl, const int8_t *__restrict__ Bsl,
     const int32_t *__restrict__ rexp, const int32_t *__restrict__ cexp, double *__restrict__ C,
     int64_t M, int64_t N, int64_t Kp, int64_t ldc, int s, int64_t nsteps, int accumulate) {
-    __shared__ __attribute__((aligned(16))) int8_t As[2][OBM * OSTR];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[2][OBN * OSTR];
+    // One contiguous LDS stage per buffer: A tile at offset 0, B tile after it.
+    __shared__ __attribute__((aligned(16))) int8_t lds[2][2 * TB * TSTR];
 
-    const int tiles_m = (int)((M + OBM - 1) / OBM);
-    const int tiles_n = (int)((N + OBN - 1) / OBN);
+    const int tiles_m = (int)((M + TB - 1) / TB);
+    const int tiles_n = (int)((N + TB - 1) / TB);
     int tm, tn;
     {
         const int nb = tiles_m * tiles_n;
         int b = blockIdx.x;
         if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
-        const int per_group = OGROUP_M * tiles_n;
+        const int per_group = TGROUP_M * tiles_n;
         const int g = b / per_group;
-        const int first_m = g * OGROUP_M;
-        const int gsize = min(tiles_m - first_m, OGROUP_M);
+        const int first_m = g * TGROUP_M;
+        const int gsize = min(tiles_m - first_m, TGROUP_M);
         const int r = b - g * per_group;
         tm = first_m + r % gsize;
         tn = r / gsize;
     }
-    const int64_t m0 = (int64_t)tm * OBM;
-    const int64_t n0 = (int64_t)tn * OBN;
+    const int64_t m0 = (int64_t)tm * TB;
+    const int64_t n0 = (int64_t)tn * TB;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -261,43 +280,46 @@ __global__ __launch_bounds__(ONT, 2) void ozaki_int8_gemm_128x128(
     const int lk = lane >> 4;
     const int lc = lane & 15;
 
-    // Load assignment: lane takes 32 bytes (two 16-byte vectors) of one slab row.
-    const int ld_row = tid >> 1;
-    const int ld_off = (tid & 1) * 32;
-    const bool a_ok = m0 + ld_row < M;
-    const bool b_ok = n0 + ld_row < N;
-    const int64_t a_row_off = (a_ok ? m0 + ld_row : 0) * Kp + ld_off;
-    const int64_t b_row_off = (b_ok ? n0 + ld_row : 0) * Kp + ld_off;
-    const int64_t a_slice_stride = M * Kp;
-    const int64_t b_slice_stride = N * Kp;
+    // Load assignment: 16-byte vector v = tid + 256*i (i < 8) covers row
+    // v / 8, byte offset (v % 8) * 16 of the BK = 128 byte stage.
+    constexpr int VPR = TBK / 16;              // vectors per row
+    constexpr int NV = TB * VPR / TNT;         // vectors per lane per operand
+    const int ld_row0 = tid / VPR;
+    const int ld_off = (tid % VPR) * 16;
+    const int64_t a_stride = M * Kp;
+    const int64_t b_stride = N * Kp;
 
-    v4i ra0, ra1, rb0, rb1;
+    v4i ra[NV], rb[NV];
     const v4i vzero = {0, 0, 0, 0};
     auto load_step = [&](int p, int q, int kt) {
-        const int8_t *pa = Asl + (int64_t)(p - 1) * a_slice_stride + a_row_off + (int64_t)kt * KT;
-        const int8_t *pb = Bsl + (int64_t)(q - 1) * b_slice_stride + b_row_off + (int64_t)kt * KT;
-        ra0 = a_ok ? *(const v4i *)pa : vzero;
-        ra1 = a_ok ? *(const v4i *)(pa + 16) : vzero;
-        rb0 = b_ok ? *(const v4i *)pb : vzero;
-        rb1 = b_ok ? *(const v4i *)(pb + 16) : vzero;
+        const int8_t *pa = Asl + (int64_t)(p - 1) * a_stride + (int64_t)kt * KT + ld_off;
+        const int8_t *pb = Bsl + (int64_t)(q - 1) * b_stride + (int64_t)kt * KT + ld_off;
+        // Rows past the matrix edge load row 0 and are zeroed by a select, so
+        // the loads need no branches.
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int64_t ra_row = m0 + ld_row0 + i * (TNT / VPR);
+            const int64_t rb_row = n0 + ld_row0 + i * (TNT / VPR);
+            const v4i va = *(const v4i *)(pa + (ra_row < M ? ra_row : 0) * Kp);
+            const v4i vb = *(const v4i *)(pb + (rb_row < N ? rb_row : 0) * Kp);
+            ra[i] = ra_row < M ? va : vzero;
+            rb[i] = rb_row < N ? vb : vzero;
+        }
     };
     auto store_step = [&](int buf) {
-        *(v4i *)&As[buf][ld_row * OSTR + ld_off] = ra0;
-        *(v4i *)&As[buf][ld_row * OSTR + ld_off + 16] = ra1;
-        *(v4i *)&Bs[buf][ld_row * OSTR + ld_off] = rb0;
-        *(v4i *)&Bs[buf][ld_row * OSTR + ld_off + 16] = rb1;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int row = ld_row0 + i * (TNT / VPR);
+            *(v4i *)&lds[buf][row * TSTR + ld_off] = ra[i];
+            *(v4i *)&lds[buf][TB * TSTR + row * TSTR + ld_off] = rb[i];
+        }
     };
 
-    v4i acc[4][4];
-    double accd[4][4][4];
+    v4i acc[8][8];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 8; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            acc[i][j] = vzero;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) accd[i][j][r] = 0.0;
-        }
+        for (int j = 0; j < 8; ++j) acc[i][j] = vzero;
 
     const int ktiles = (int)(Kp / KT);
     StepIter it;
@@ -305,64 +327,90 @@ __global__ __launch_bounds__(ONT, 2) void ozaki_int8_gemm_128x128(
     load_step(it.p, it.t - it.p, it.kt);
     store_step(0);
     __syncthreads();
+    bool first_fold = true;
 
     for (int64_t step = 0; step < nsteps; ++step) {
         const int buf = (int)(step & 1);
         const int t_cur = it.t;
-        const bool flush = it.advance();
+        const bool fold = it.advance();
         const bool more = step + 1 < nsteps;
         if (more) load_step(it.p, it.t - it.p, it.kt);
 
-        v4i af[4], bf[4];
+#pragma unroll 1
+        for (int kk = 0; kk < TBK / KT; ++kk) {
+            v4i af[8], bf[8];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            af[i] = *(const v4i *)&As[buf][(wm * 64 + i * 16 + lc) * OSTR + 16 * lk];
+            for (int i = 0; i < 8; ++i)
+                af[i] = *(const v4i *)&lds[buf][(wm * 128 + i * 16 + lc) * TSTR + kk * KT + 16 * lk];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            bf[j] = *(const v4i *)&Bs[buf][(wn * 64 + j * 16 + lc) * OSTR + 16 * lk];
+            for (int j = 0; j < 8; ++j)
+                bf[j] = *(const v4i *)&lds[buf][TB * TSTR + (wn * 128 + j * 16 + lc) * TSTR + kk * KT +
+                                                16 * lk];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 8; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
-
-        if (flush) {
-            const double w = ldexp(1.0, -7 * t_cur);
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        accd[i][j][r] = fma((double)acc[i][j][r], w, accd[i][j][r]);
-                    acc[i][j] = vzero;
-                }
+                for (int j = 0; j < 8; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
         }
+
+        // The prefetched slices go to the idle LDS buffer before the fold, so
+        // they do not stay live across it.
         if (more) store_step(buf ^ 1);
-        __syncthreads();
-    }
 
+        if (fold) {
+            // Fold through LDS: the buffer read by this step is free once every
+            // wave has passed the barrier.  Four passes; in pass q each wave
+            // dumps its INT32 accumulators of fragment rows 2q, 2q+1 (32 rows
+            // x 128 columns, rows padded to FOLD_STR ints) and then folds them
+            // into C row by row, two adjacent columns per lane.
+            __syncthreads();
+            int *fold_buf = (int *)&lds[buf][0] + wave * (32 * FOLD_STR);
+            const int64_t fcol = n0 + wn * 128 + 2 * lane;
+            int ec0 = 0, ec1 = 0;
+            if (fcol < N) ec0 = cexp[fcol];
+            if (fcol + 1 < N) ec1 = cexp[fcol + 1];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+            for (int q = 0; q < 4; ++q) {
+                // INT32 layout: lane l, register r holds row 4*(l/16)+r, column l%16.
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int64_t row = m0 + wm * 64 + i * 16 + 4 * lk + r;
-            if (row >= M) continue;
-            const int er = rexp[row];
-            double *c_row = C + row * ldc;
+                for (int ii = 0; ii < 2; ++ii)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int64_t col = n0 + wn * 64 + j * 16 + lc;
-                if (col >= N) continue;
-                const int ec = cexp[col];
-                double v;
-                if (er == EXP_BAD || ec == EXP_BAD)
-                    v = __longlong_as_double(0x7ff8000000000000ll);
-                else
-                    v = ldexp(accd[i][j][r], er + ec);
-                c_row[col] = accumulate ? c_row[col] + v : v;
+                    for (int j = 0; j < 8; ++j)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            fold_buf[(ii * 16 + 4 * lk + r) * FOLD_STR + j * 16 + lc] =
+                                acc[2 * q + ii][j][r];
+                __syncthreads();
+#pragma unroll 2
+                for (int rho = 0; rho < 32; ++rho) {
+                    const int64_t row = m0 + wm * 128 + 32 * q + rho;
+                    if (row >= M) break;
+                    const int er = rexp[row];
+                    const int g0 = fold_buf[rho * FOLD_STR + 2 * lane];
+                    const int g1 = fold_buf[rho * FOLD_STR + 2 * lane + 1];
+                    double *c_row = C + row * ldc;
+                    if (fcol < N) {
+                        const double x = (er == EXP_BAD || ec0 == EXP_BAD)
+                                             ? __longlong_as_double(0x7ff8000000000000ll)
+                                             : ldexp((double)g0, er + ec0 - 7 * t_cur);
+                        c_row[fcol] = (first_fold && !accumulate) ? x : c_row[fcol] + x;
+                    }
+                    if (fcol + 1 < N) {
+                        const double x = (er == EXP_BAD || ec1 == EXP_BAD)
+                                             ? __longlong_as_double(0x7ff8000000000000ll)
+                                             : ldexp((double)g1, er + ec1 - 7 * t_cur);
+                        c_row[fcol + 1] = (first_fold && !accumulate) ? x : c_row[fcol + 1] + x;
+                    }
+                }
+                __syncthreads();
             }
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[i][j] = vzero;
+            first_fold = false;
         }
+        __syncthreads();
     }
 }
 
@@ -447,7 +495,7 @@ static int split_gemm_impl(int form, int slices, const double *A, const double *
     MGX_ARG_CHECK(A != nullptr && B != nullptr, "A or B is NULL");
     MGX_ARG_CHECK(lda >= k && ldb >= n, "lda < k or ldb < n");
     const SplitLayout L = split_layout(slices, m, n, k);
-    const int64_t tiles = ((m + OBM - 1) / OBM) * ((n + OBN - 1) / OBN);
+    const int64_t tiles = ((m + TB - 1) / TB) * ((n + TB - 1) / TB);
     MGX_ARG_CHECK(tiles <= 0x7fffffffll, "matrix too large for one launch");
 
     char *ws = (char *)workspace;
@@ -507,10 +555,10 @@ static int split_gemm_impl(int form, int slices, const double *A, const double *
         if ((rc = mgx::launch_status("slice_cols_kernel")) != MGX_OK) return rc;
     }
     if ((rc = mark(2)) != MGX_OK) return rc;
-    const int64_t nsteps = (L.Kp / KT) * (int64_t)(slices * (slices + 1) / 2);
-    hipLaunchKernelGGL(ozaki_int8_gemm_128x128, dim3((unsigned)tiles), dim3(ONT), 0, st, Asl, Bsl,
+    const int64_t nsteps = (L.Kp / TBK) * (int64_t)(slices * (slices + 1) / 2);
+    hipLaunchKernelGGL(ozaki_int8_gemm_256x256, dim3((unsigned)tiles), dim3(TNT), 0, st, Asl, Bsl,
                        rexp, cexp, C, m, n, L.Kp, ldc, slices, nsteps, accumulate ? 1 : 0);
-    if ((rc = mgx::launch_status("ozaki_int8_gemm_128x128")) != MGX_OK) return rc;
+    if ((rc = mgx::launch_status("ozaki_int8_gemm_256x256")) != MGX_OK) return rc;
     return mark(3);
 }
 

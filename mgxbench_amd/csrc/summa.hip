@@ -1,0 +1,223 @@
+// SUMMA panel streaming: the single-GPU stand-in for the reference's per-panel
+// row/column broadcasts (2D-block-matrix/mmb.cu:239-361).
+//
+// One rank's local update C_ij += sum_p A_ik(p) . B_kj(p) over panels of
+// width `panel` (ragged last panel, 2D-block-matrix/mmb.cu:242-246).  Panel
+// p of A is an m x kc column block of the host A strip, panel p of B is kc
+// rows of the host B strip.  They reach the device by hipMemcpy2DAsync /
+// hipMemcpyAsync on a transfer stream; the compute stream waits on a
+// per-buffer "panel ready" event and runs the accumulate product; a
+// "compute done" event guards reuse of each of the two device buffers.
+//
+// MI355X-native differences from the reference's loop:
+//  * With pinned host strips the A panel is gathered by the DMA engine
+//    (strided 2D copy) instead of host memcpy packing; with pageable strips
+//    the library packs into its own pinned double buffer as the reference
+//    does (pack_a / pack_b, 2D-block-matrix/mmb.cu:60-72).
+//  * Buffer reuse is ordered by stream waits on events, not host syncs, so
+//    the whole loop can be captured into one hipGraph and launched once.
+#include "mgx_internal.hpp"
+
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+extern "C" int mgx_split_gemm(int form, int slices, const double *A, const double *B, double *C,
+                              int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb,
+                              int64_t ldc, int accumulate, void *workspace,
+                              size_t workspace_bytes, void *stream);
+
+namespace {
+
+struct SummaResources {
+    hipStream_t compute = nullptr, transfer = nullptr;
+    hipEvent_t ready[2] = {nullptr, nullptr}, done[2] = {nullptr, nullptr};
+    hipEvent_t fork = nullptr, join = nullptr;
+    double *dA[2] = {nullptr, nullptr}, *dB[2] = {nullptr, nullptr};
+    double *hA[2] = {nullptr, nullptr}, *hB[2] = {nullptr, nullptr};
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    ~SummaResources() {
+        if (compute) (void)hipStreamSynchronize(compute);
+        if (transfer) (void)hipStreamSynchronize(transfer);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (int i = 0; i < 2; ++i) {
+            if (ready[i]) (void)hipEventDestroy(ready[i]);
+            if (done[i]) (void)hipEventDestroy(done[i]);
+            if (dA[i]) (void)hipFree(dA[i]);
+            if (dB[i]) (void)hipFree(dB[i]);
+            if (hA[i]) (void)hipHostFree(hA[i]);
+            if (hB[i]) (void)hipHostFree(hB[i]);
+        }
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (ws) (void)hipFree(ws);
+        if (compute) (void)hipStreamDestroy(compute);
+        if (transfer) (void)hipStreamDestroy(transfer);
+    }
+};
+
+bool host_is_pinned(const void *p) {
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeHost;
+}
+
+}  // namespace
+
+extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_t lda,
+                             const double *hB, int64_t ldb, double *dC, int64_t ldc,
+                             mgx_summa_stats *stats) {
+    mgx::clear_error();
+    MGX_ARG_CHECK(P != nullptr, "params is NULL");
+    const int64_t m = P->m, n = P->n, k = P->k, bk = P->panel;
+    MGX_ARG_CHECK(m > 0 && n > 0 && k > 0 && bk > 0, "sizes and panel width must be positive");
+    MGX_ARG_CHECK(hA && hB && dC, "NULL matrix pointer");
+    MGX_ARG_CHECK(lda >= k && ldb >= n && ldc >= n, "leading dimension too small");
+    MGX_ARG_CHECK(P->form == MGX_FORM_FP64 || P->form == MGX_FORM_INT8, "unsupported form");
+    MGX_ARG_CHECK(P->copy_mode == MGX_SUMMA_COPY_ASYNC || P->copy_mode == MGX_SUMMA_COPY_HOST_SYNC,
+                  "unknown copy mode");
+    const int64_t panel = bk < k ? bk : k;
+    const int64_t npanels = (k + panel - 1) / panel;
+    const bool pinned = host_is_pinned(hA) && host_is_pinned(hB);
+    const bool graph = P->use_graph != 0;
+    MGX_ARG_CHECK(!graph || (pinned && P->copy_mode == MGX_SUMMA_COPY_ASYNC && !P->serialize),
+                  "hipGraph capture needs pinned host strips, async copies and no serialization");
+
+    SummaResources R;
+    MGX_HIP_TRY(hipStreamCreateWithFlags(&R.compute, hipStreamNonBlocking));
+    MGX_HIP_TRY(hipStreamCreateWithFlags(&R.transfer, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+        MGX_HIP_TRY(hipEventCreateWithFlags(&R.ready[i], hipEventDisableTiming));
+        MGX_HIP_TRY(hipEventCreateWithFlags(&R.done[i], hipEventDisableTiming));
+        if (hipMalloc(&R.dA[i], sizeof(double) * m * panel) != hipSuccess ||
+            hipMalloc(&R.dB[i], sizeof(double) * panel * n) != hipSuccess)
+            return mgx::set_error(MGX_ERR_ALLOC, "panel buffer allocation failed");
+        if (!pinned) {
+            if (hipHostMalloc((void **)&R.hA[i], sizeof(double) * m * panel) != hipSuccess ||
+                hipHostMalloc((void **)&R.hB[i], sizeof(double) * panel * n) != hipSuccess)
+                return mgx::set_error(MGX_ERR_ALLOC, "pinned staging allocation failed");
+        }
+    }
+    MGX_HIP_TRY(hipEventCreateWithFlags(&R.fork, hipEventDisableTiming));
+    MGX_HIP_TRY(hipEventCreateWithFlags(&R.join, hipEventDisableTiming));
+    if (P->form == MGX_FORM_INT8) {
+        R.ws_bytes = mgx_split_workspace_bytes(MGX_FORM_INT8, P->slices, m, n, panel);
+        if (R.ws_bytes == 0) return mgx::set_error(MGX_ERR_UNSUPPORTED, "slice count out of range");
+        if (hipMalloc(&R.ws, R.ws_bytes) != hipSuccess)
+            return mgx::set_error(MGX_ERR_ALLOC, "split workspace allocation failed");
+    }
+
+    // Enqueues the panel loop on the two streams.  When `host_sync` is set
+    // (pageable strips, or a copy mode that waits on the host), the host
+    // waits for buffer reuse and packs panels itself.
+    auto enqueue = [&](bool capturing) -> int {
+        int rc;
+        MGX_HIP_TRY(hipEventRecord(R.fork, R.compute));
+        MGX_HIP_TRY(hipStreamWaitEvent(R.transfer, R.fork, 0));
+        for (int64_t it = 0; it < npanels; ++it) {
+            const int cur = (int)(it & 1);
+            const int64_t k0 = it * panel;
+            const int64_t kc = (k - k0) < panel ? (k - k0) : panel;
+            if (it >= 2) {
+                if (!pinned && !capturing) {
+                    MGX_HIP_TRY(hipEventSynchronize(R.done[cur]));
+                } else {
+                    MGX_HIP_TRY(hipStreamWaitEvent(R.transfer, R.done[cur], 0));
+                }
+            }
+            const double *srcA = hA + k0;
+            const double *srcB = hB + k0 * ldb;
+            int64_t spitchA = lda, spitchB = ldb;
+            if (!pinned) {
+                // pack_a / pack_b into the pinned staging buffer
+                for (int64_t i = 0; i < m; ++i)
+                    std::memcpy(R.hA[cur] + i * kc, hA + i * lda + k0, sizeof(double) * kc);
+                for (int64_t r = 0; r < kc; ++r)
+                    std::memcpy(R.hB[cur] + r * n, hB + (k0 + r) * ldb, sizeof(double) * n);
+                srcA = R.hA[cur];
+                srcB = R.hB[cur];
+                spitchA = kc;
+                spitchB = n;
+            }
+            MGX_HIP_TRY(hipMemcpy2DAsync(R.dA[cur], sizeof(double) * kc, srcA,
+                                         sizeof(double) * spitchA, sizeof(double) * kc, m,
+                                         hipMemcpyHostToDevice, R.transfer));
+            MGX_HIP_TRY(hipMemcpy2DAsync(R.dB[cur], sizeof(double) * n, srcB,
+                                         sizeof(double) * spitchB, sizeof(double) * n, kc,
+                                         hipMemcpyHostToDevice, R.transfer));
+            if (P->copy_mode == MGX_SUMMA_COPY_HOST_SYNC || P->serialize)
+                MGX_HIP_TRY(hipStreamSynchronize(R.transfer));
+            MGX_HIP_TRY(hipEventRecord(R.ready[cur], R.transfer));
+            MGX_HIP_TRY(hipStreamWaitEvent(R.compute, R.ready[cur], 0));
+            if (P->form == MGX_FORM_FP64) {
+                rc = mgx_dgemm_accumulate(R.dA[cur], R.dB[cur], dC, m, n, kc, 0, kc, kc, n, ldc,
+                                          32, R.compute);
+            } else {
+                rc = mgx_split_gemm(MGX_FORM_INT8, P->slices, R.dA[cur], R.dB[cur], dC, m, n, kc,
+                                    kc, n, ldc, 1, R.ws, R.ws_bytes, R.compute);
+            }
+            if (rc != MGX_OK) return rc;
+            MGX_HIP_TRY(hipEventRecord(R.done[cur], R.compute));
+            if (P->serialize) MGX_HIP_TRY(hipStreamSynchronize(R.compute));
+        }
+        MGX_HIP_TRY(hipEventRecord(R.join, R.transfer));
+        MGX_HIP_TRY(hipStreamWaitEvent(R.compute, R.join, 0));
+        return MGX_OK;
+    };
+
+    int rc;
+    if (graph) {
+        MGX_HIP_TRY(hipStreamBeginCapture(R.compute, hipStreamCaptureModeThreadLocal));
+        rc = enqueue(true);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(R.compute, &g);
+        R.graph = g;
+        if (rc != MGX_OK) return rc;
+        if (e != hipSuccess)
+            return mgx::set_error(MGX_ERR_HIP, std::string("hipStreamEndCapture failed: ") +
+                                                   hipGetErrorString(e));
+        MGX_HIP_TRY(hipGraphInstantiate(&R.exec, R.graph, nullptr, nullptr, 0));
+        MGX_HIP_TRY(hipGraphUpload(R.exec, R.compute));
+    }
+    MGX_HIP_TRY(hipDeviceSynchronize());
+    const auto t0 = std::chrono::steady_clock::now();
+    if (graph) {
+        MGX_HIP_TRY(hipGraphLaunch(R.exec, R.compute));
+    } else {
+        if ((rc = enqueue(false)) != MGX_OK) return rc;
+    }
+    MGX_HIP_TRY(hipStreamSynchronize(R.compute));
+    const auto t1 = std::chrono::steady_clock::now();
+    if (stats) {
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+        stats->panels = npanels;
+        stats->host_pinned = pinned ? 1 : 0;
+    }
+    return MGX_OK;
+}
+
+extern "C" int mgx_host_alloc(void **ptr, size_t bytes) {
+    mgx::clear_error();
+    MGX_ARG_CHECK(ptr != nullptr, "ptr is NULL");
+    *ptr = nullptr;
+    if (bytes == 0) return MGX_OK;
+    if (hipHostMalloc(ptr, bytes) != hipSuccess) {
+        *ptr = nullptr;
+        return mgx::set_error(MGX_ERR_ALLOC, "hipHostMalloc failed");
+    }
+    return MGX_OK;
+}
+
+extern "C" int mgx_host_free(void *ptr) {
+    mgx::clear_error();
+    if (ptr) MGX_HIP_TRY(hipHostFree(ptr));
+    return MGX_OK;
+}

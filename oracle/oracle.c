@@ -26,7 +26,7 @@
  *    (2D-block-matrix/mmb.cu:381-385).
  *  - orc_split_*: a bit-exact model of libmgx's INT8 split-precision
  *    (Ozaki scheme I) product: exponents, slices, exact group sums with the
- *    same INT32 flush chunks, and the same FP64 reconstruction sequence.
+ *    same INT32 flush chunks, and the same FP64 fold sequence into C.
  *
  * Parity pins (tests/test_oracle.py): exact rational brute force on tiny
  * inputs for the fma chains, the accumulate form, the double-double dot and
@@ -320,7 +320,9 @@ ORC_API int orc_split_chunk_tiles(int s, int t) {
     const int pmax = t - 1 < s ? t - 1 : s;
     const long long np = pmax - pmin + 1;
     long long c = 2147483647ll / (np * 127ll * 127ll * ORC_KT);
-    return (int)(c < 1 ? 1 : c);
+    /* whole 128-byte LDS stages of the product kernel */
+    c = (c / 2 < 1 ? 1 : c / 2) * 2;
+    return (int)c;
 }
 
 /* Full model: C = A.B (or C += A.B) exactly as the INT8 split kernel does. */
@@ -364,12 +366,15 @@ ORC_API int orc_split_gemm_int8(int s, const double *A, const double *B, double 
 #pragma omp parallel for schedule(dynamic, 1) collapse(2) reduction(| : overflow)
     for (int64_t i = 0; i < m; ++i) {
         for (int64_t j = 0; j < n; ++j) {
-            double v = 0.0;
+            /* Fold sequence of the product kernel: x = ldexp(G, e_i+f_j-7t) per
+             * (group, chunk); C = x (or C_in + x) first, then C = C + x. */
+            double v = accumulate ? C[i * ldc + j] : 0.0;
+            int first = 1;
+            const int bad = rexp[i] == ORC_EXP_BAD || cexp[j] == ORC_EXP_BAD;
             for (int t = 2; t <= s + 1; ++t) {
                 const int pmin = t - s > 1 ? t - s : 1;
                 const int pmax = t - 1 < s ? t - 1 : s;
                 const int64_t chunk = orc_split_chunk_tiles(s, t);
-                const double w = ldexp(1.0, -7 * t);
                 for (int64_t c0 = 0; c0 < ktiles; c0 += chunk) {
                     int64_t k0 = c0 * ORC_KT;
                     int64_t k1 = (c0 + chunk) * ORC_KT;
@@ -381,15 +386,15 @@ ORC_API int orc_split_gemm_int8(int s, const double *A, const double *B, double 
                         for (int64_t kk = k0; kk < k1; ++kk) g += (long long)ar[kk] * br[kk];
                     }
                     if (g > 2147483647ll || g < -2147483648ll) overflow = 1;
-                    v = fma((double)g, w, v);
+                    const double x = bad ? NAN : ldexp((double)g, rexp[i] + cexp[j] - 7 * t);
+                    if (first)
+                        v = accumulate ? v + x : x;
+                    else
+                        v = v + x;
+                    first = 0;
                 }
             }
-            double res;
-            if (rexp[i] == ORC_EXP_BAD || cexp[j] == ORC_EXP_BAD)
-                res = NAN;
-            else
-                res = ldexp(v, rexp[i] + cexp[j]);
-            C[i * ldc + j] = accumulate ? C[i * ldc + j] + res : res;
+            C[i * ldc + j] = v;
         }
     }
     free(rexp); free(cexp); free(As); free(Bs);

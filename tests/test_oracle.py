@@ -213,8 +213,9 @@ def test_split_chunk_respects_int32(orc):
         for t in range(2, s + 2):
             pairs = min(s, t - 1) - max(1, t - s) + 1
             c = orc.split_chunk_tiles(s, t)
+            assert c % 2 == 0          # whole 128-byte stages
             assert pairs * c * 64 * 127 * 127 <= 2 ** 31 - 1
-            assert pairs * (c + 1) * 64 * 127 * 127 > 2 ** 31 - 1
+            assert pairs * (c + 2) * 64 * 127 * 127 > 2 ** 31 - 1
 
 
 def _py_split_product(A, B, s):
@@ -230,14 +231,15 @@ def _py_split_product(A, B, s):
     C = np.empty((m, n))
     for i in range(m):
         for j in range(n):
-            v = 0.0
+            v = None
             for t in range(2, s + 2):
                 g = 0
                 for p in range(max(1, t - s), min(s, t - 1) + 1):
                     q = t - p
                     g += sum(SA[i][kk][p - 1] * SB[j][kk][q - 1] for kk in range(k))
-                v = fr.fma(float(g), 2.0 ** (-7 * t), v)
-            C[i, j] = fr.rnd(Fraction(v) * Fraction(2) ** (re[i] + ce[j]))
+                x = fr.rnd(Fraction(g) * Fraction(2) ** (re[i] + ce[j] - 7 * t))
+                v = x if v is None else fr.add(v, x)
+            C[i, j] = v
     return C
 
 
@@ -283,9 +285,14 @@ def test_split_model_accumulate_and_nonfinite(orc):
     A = rng_matrix(5, 40, 101)
     B = rng_matrix(40, 6, 102)
     C0 = rng_matrix(5, 6, 103)
-    plain = orc.split_gemm_int8(A, B, 6)
-    acc = orc.split_gemm_int8(A, B, 6, C=C0)
-    assert np.array_equal(acc, C0 + plain)
+    # accumulate form: C_in is added before the first fold (exact here: with
+    # 7-bit integer inputs every fold is exact)
+    r = np.random.default_rng(8)
+    Ai = r.integers(-127, 128, size=(5, 40)).astype(np.float64)
+    Bi = r.integers(-127, 128, size=(40, 6)).astype(np.float64)
+    Ci = r.integers(-1000, 1000, size=(5, 6)).astype(np.float64)
+    acc = orc.split_gemm_int8(Ai, Bi, 6, C=Ci)
+    assert np.array_equal(acc, Ci + Ai @ Bi)
     A[2, 7] = np.inf
     B[3, 4] = np.nan
     out = orc.split_gemm_int8(A, B, 6)
