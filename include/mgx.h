@@ -127,6 +127,25 @@ int mgx_split_gemm_events(int form, int slices,
 /* Supported slice-count range for a split form (inclusive). */
 int mgx_split_slices_range(int form, int *min_slices, int *max_slices);
 
+/* ---- accuracy report ----------------------------------------------------
+ * Error of a computed device product C (m x n) against a double-double
+ * reference of A.B evaluated on the GPU for the listed rows (host array of
+ * `nrows` row indices): normwise = ||C - AB||_F / ||AB||_F over those rows,
+ * componentwise = max |C - AB| / (|A||B|).  A NaN in C reports Inf.
+ * Synchronous.  No counterpart in the reference (its only check is the
+ * constant known answer); it backs the error statement of every result. */
+typedef struct mgx_error_stats {
+    double normwise;
+    double componentwise;
+    double max_abs;
+    int64_t rows_checked;
+} mgx_error_stats;
+
+int mgx_error_report(const double *A, const double *B, const double *C,
+                     int64_t m, int64_t n, int64_t k,
+                     int64_t lda, int64_t ldb, int64_t ldc,
+                     const int64_t *rows, int64_t nrows, mgx_error_stats *out);
+
 /* ---- SUMMA panel streaming ----------------------------------------------
  * Replaces the local part of the 2D SUMMA panel loop
  * (2D-block-matrix/mmb.cu:239-371): for panels of width `panel` (ragged last

@@ -21,7 +21,7 @@ from ._lib import FORM_BF16, FORM_FP64, FORM_INT8, MgxError, lib
 
 __all__ = [
     "ABMultiply", "ABMultiplyAccumulateAsync", "validate_matrix_C", "split_gemm",
-    "split_gemm_events", "split_workspace_bytes", "fill", "fill_exponents", "scale_rows_cols", "version", "peak",
+    "split_gemm_events", "split_workspace_bytes", "error_report", "summa_run", "fill", "fill_exponents", "scale_rows_cols", "version", "peak",
     "FORM_FP64", "FORM_INT8", "FORM_BF16", "MgxError", "lib",
 ]
 
@@ -97,6 +97,32 @@ def split_gemm_events(a, b, c, m, n, k, lda, ldb, ldc, events, *, form=FORM_INT8
     _lib.call("mgx_split_gemm_events", form, slices, _ptr(a), _ptr(b), _ptr(c), m, n, k, lda,
               ldb, ldc, 1 if accumulate else 0, _ptr(workspace), workspace_bytes,
               _stream(stream), handles)
+
+
+def error_report(a, b, c, m, n, k, lda, ldb, ldc, rows) -> dict:
+    """Normwise / componentwise error of C against a GPU double-double A.B on `rows`."""
+    import ctypes
+    import numpy as np
+    r = np.ascontiguousarray(np.asarray(rows, dtype=np.int64))
+    out = _lib.ErrorStats()
+    _lib.call("mgx_error_report", _ptr(a), _ptr(b), _ptr(c), m, n, k, lda, ldb, ldc,
+              r.ctypes.data, len(r), ctypes.byref(out))
+    return {"normwise": out.normwise, "componentwise": out.componentwise,
+            "max_abs": out.max_abs, "rows_checked": out.rows_checked}
+
+
+def summa_run(h_a, lda, h_b, ldb, d_c, ldc, *, m, n, k, panel, form=FORM_FP64, slices=8,
+              copy_mode=0, use_graph=False, serialize=False) -> dict:
+    """One rank's SUMMA panel loop: C += sum_p A_p . B_p with panels streamed
+    from host strips (pointers as ints or CPU tensors; pinned ones are
+    gathered by DMA directly).  Returns the loop time and panel count."""
+    import ctypes
+    P = _lib.SummaParams(m, n, k, panel, form, slices, copy_mode, 1 if use_graph else 0,
+                         1 if serialize else 0)
+    S = _lib.SummaStats()
+    _lib.call("mgx_summa_run", ctypes.byref(P), _ptr(h_a), lda, _ptr(h_b), ldb, _ptr(d_c), ldc,
+              ctypes.byref(S))
+    return {"seconds": S.seconds, "panels": S.panels, "host_pinned": S.host_pinned}
 
 
 def fill(x, rows, cols, ld, kind, seed=0, value=0.0, stream=None) -> None:

@@ -29,6 +29,23 @@ _int = ctypes.c_int
 _dbl = ctypes.c_double
 _u64 = ctypes.c_uint64
 
+class ErrorStats(ctypes.Structure):
+    _fields_ = [("normwise", ctypes.c_double), ("componentwise", ctypes.c_double),
+                ("max_abs", ctypes.c_double), ("rows_checked", ctypes.c_int64)]
+
+
+class SummaParams(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_int64), ("n", ctypes.c_int64), ("k", ctypes.c_int64),
+                ("panel", ctypes.c_int64), ("form", ctypes.c_int), ("slices", ctypes.c_int),
+                ("copy_mode", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("serialize", ctypes.c_int)]
+
+
+class SummaStats(ctypes.Structure):
+    _fields_ = [("seconds", ctypes.c_double), ("panels", ctypes.c_int64),
+                ("host_pinned", ctypes.c_int)]
+
+
 _SIGNATURES = {
     "mgx_last_error": (ctypes.c_char_p, []),
     "mgx_version": (ctypes.c_char_p, []),
@@ -46,6 +63,12 @@ _SIGNATURES = {
                                      _i64, _int, _vp, ctypes.c_size_t, _vp,
                                      ctypes.POINTER(_vp)]),
     "mgx_split_slices_range": (_int, [_int, ctypes.POINTER(_int), ctypes.POINTER(_int)]),
+    "mgx_error_report": (_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _i64,
+                                ctypes.POINTER(ErrorStats)]),
+    "mgx_summa_run": (_int, [ctypes.POINTER(SummaParams), _vp, _i64, _vp, _i64, _vp, _i64,
+                             ctypes.POINTER(SummaStats)]),
+    "mgx_host_alloc": (_int, [ctypes.POINTER(_vp), ctypes.c_size_t]),
+    "mgx_host_free": (_int, [_vp]),
     "mgx_fill": (_int, [_vp, _i64, _i64, _i64, _int, _u64, _dbl, _vp]),
     "mgx_fill_exponents": (_int, [_vp, _i64, _u64, _int, _int, _vp]),
     "mgx_scale_rows_cols": (_int, [_vp, _i64, _i64, _i64, _vp, _vp, _vp]),

@@ -23,8 +23,8 @@ BIN = PKG / "bin"
 LIB = PKG / "libmgx.so"
 INCLUDE = PKG.parent / "include"
 
-LIB_SOURCES = ["runtime.hip", "dgemm.hip", "ozaki.hip", "summa.hip"]
-EXE_SOURCES = {}  # name -> source file in csrc/, linked against libmgx.so
+LIB_SOURCES = ["runtime.hip", "dgemm.hip", "ozaki.hip", "summa.hip", "accuracy.hip"]
+EXE_SOURCES = {"mm_1d": "mm_1d.cpp", "mm_2d": "mm_2d.cpp"}  # linked against libmgx.so
 
 ARCH = "gfx950"
 COMMON_FLAGS = [
@@ -73,10 +73,14 @@ def build(jobs: int = 8) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, LIB)
+    exe_cmds = []
     for name, src in EXE_SOURCES.items():
         exe = BIN / name
-        _run([cc, *COMMON_FLAGS, str(CSRC / src), "-o", str(exe), f"-L{PKG}", "-lmgx",
-              f"-Wl,-rpath,$ORIGIN/.."])
+        exe_cmds.append([cc, "-x", "hip", *COMMON_FLAGS, str(CSRC / src), "-o", str(exe),
+                         f"-L{PKG}", "-lmgx", "-Wl,-rpath,$ORIGIN/.."])
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        for f in [ex.submit(_run, c) for c in exe_cmds]:
+            f.result()
     return LIB
 
 

@@ -1,0 +1,176 @@
+"""SUMMA panel streaming (mgx_summa_run) and the GPU accuracy report, through
+the C ABI.
+
+The panel loop must equal the reference's: C += A[:, k0:k0+kc] . B[k0:k0+kc, :]
+per panel in order (2D-block-matrix/mmb.cu:239-361), ragged last panel
+included.  Integer inputs make every order exact, so those cases compare bit
+for bit; real inputs are compared with the per-panel reference-order oracle
+within 2 gamma_kc |A||B| per panel.  Overlapped and serialized runs, graph and
+stream runs, pinned and pageable strips must agree bit for bit.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests.gpu_util import announce, empty, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+class PinnedArray:
+    """A numpy view of library-allocated pinned host memory."""
+
+    def __init__(self, mgx, rows, cols):
+        self.mgx = mgx
+        ptr = ctypes.c_void_p()
+        mgx._lib.call("mgx_host_alloc", ctypes.byref(ptr), rows * cols * 8)
+        self.ptr = ptr.value
+        buf = (ctypes.c_double * (rows * cols)).from_address(self.ptr)
+        self.array = np.ctypeslib.as_array(buf).reshape(rows, cols)
+
+    def free(self):
+        self.mgx._lib.call("mgx_host_free", self.ptr)
+
+
+def _run(mgx, A, B, panel, *, pinned, form=0, slices=8, graph=False, serialize=False,
+         copy_mode=0, C0=None):
+    m, k = A.shape
+    n = B.shape[1]
+    dC = to_dev(C0) if C0 is not None else empty(m, n, fill=0.0)
+    if pinned:
+        pa, pb = PinnedArray(mgx, m, k), PinnedArray(mgx, k, n)
+        pa.array[:] = A
+        pb.array[:] = B
+        try:
+            st = mgx.summa_run(pa.ptr, k, pb.ptr, n, dC, n, m=m, n=n, k=k, panel=panel, form=form,
+                               slices=slices, use_graph=graph, serialize=serialize,
+                               copy_mode=copy_mode)
+        finally:
+            pa.free()
+            pb.free()
+    else:
+        A = np.ascontiguousarray(A)
+        B = np.ascontiguousarray(B)
+        st = mgx.summa_run(A.ctypes.data, k, B.ctypes.data, n, dC, n, m=m, n=n, k=k, panel=panel,
+                           form=form, slices=slices, serialize=serialize, copy_mode=copy_mode)
+    assert st["panels"] == (k + min(panel, k) - 1) // min(panel, k)
+    assert st["host_pinned"] == (1 if pinned else 0)
+    return to_host(dC), st
+
+
+@pytest.mark.parametrize("pinned,graph", [(True, False), (True, True), (False, False)])
+def test_integer_panels_exact(mgx, pinned, graph):
+    r = np.random.default_rng(1)
+    m, n, k, panel = 300, 260, 1000, 256      # ragged last panel of 232
+    announce(f"summa integer pinned={pinned} graph={graph}")
+    A = r.integers(-60, 60, size=(m, k)).astype(np.float64)
+    B = r.integers(-60, 60, size=(k, n)).astype(np.float64)
+    C, st = _run(mgx, A, B, panel, pinned=pinned, graph=graph)
+    assert np.array_equal(C, (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float64))
+
+
+def test_real_panels_match_reference_order(mgx, orc):
+    m, n, k, panel = 257, 199, 700, 128
+    announce("summa uniform reference order")
+    A = orc.fill(m, k, 1, seed=11)
+    B = orc.fill(k, n, 1, seed=12)
+    C, _ = _run(mgx, A, B, panel, pinned=True)
+    ref = np.zeros((m, n))
+    bound = np.zeros((m, n))
+    u = 2.0 ** -53
+    for k0 in range(0, k, panel):
+        kc = min(panel, k - k0)
+        Ap = np.ascontiguousarray(A[:, k0:k0 + kc])
+        Bp = np.ascontiguousarray(B[k0:k0 + kc])
+        # packed panel, lda = kc (2D-block-matrix/mmb.cu:356-358)
+        ref = orc.gemm_accumulate_order(Ap, Bp, ref, kc, 0, kc)
+        absp = orc.gemm_abs_rows(Ap, Bp)
+        bound += 2 * (kc * u / (1 - kc * u)) * absp + 4 * u * (np.abs(ref) + absp)
+    assert np.all(np.abs(C - ref) <= bound)
+
+
+def test_overlapped_equals_serialized_and_graph(mgx, orc):
+    m, n, k, panel = 320, 288, 1500, 512
+    announce("summa overlap vs serialize vs graph")
+    A = orc.fill(m, k, 1, seed=21)
+    B = orc.fill(k, n, 1, seed=22)
+    C_over, _ = _run(mgx, A, B, panel, pinned=True)
+    C_ser, _ = _run(mgx, A, B, panel, pinned=True, serialize=True)
+    C_graph, _ = _run(mgx, A, B, panel, pinned=True, graph=True)
+    C_page, _ = _run(mgx, A, B, panel, pinned=False)
+    C_sync, _ = _run(mgx, A, B, panel, pinned=True, copy_mode=1)
+    for other in (C_ser, C_graph, C_page, C_sync):
+        assert np.array_equal(C_over, other)
+
+
+def test_graph_replays_are_identical(mgx, orc):
+    m, n, k, panel = 200, 200, 600, 256
+    announce("summa graph repeat")
+    A = orc.fill(m, k, 1, seed=31)
+    B = orc.fill(k, n, 1, seed=32)
+    first, _ = _run(mgx, A, B, panel, pinned=True, graph=True)
+    for _ in range(2):
+        again, _ = _run(mgx, A, B, panel, pinned=True, graph=True)
+        assert np.array_equal(first, again)
+
+
+def test_int8_form_panels_bit_exact(mgx, orc):
+    m, n, k, panel = 190, 170, 600, 256
+    announce("summa int8 form")
+    A = orc.fill(m, k, 1, seed=41)
+    B = orc.fill(k, n, 1, seed=42)
+    C, _ = _run(mgx, A, B, panel, pinned=True, form=mgx.FORM_INT8, slices=8)
+    ref = np.zeros((m, n))
+    for k0 in range(0, k, panel):
+        kc = min(panel, k - k0)
+        ref = orc.split_gemm_int8(np.ascontiguousarray(A[:, k0:k0 + kc]),
+                                  np.ascontiguousarray(B[k0:k0 + kc]), 8, C=ref)
+    assert np.array_equal(C, ref)
+
+
+def test_invalid_summa_arguments(mgx):
+    A = np.zeros((4, 4))
+    dC = empty(4, 4)
+    with pytest.raises(mgx.MgxError):      # graph capture needs pinned strips
+        mgx.summa_run(A.ctypes.data, 4, A.ctypes.data, 4, dC, 4, m=4, n=4, k=4, panel=2,
+                      use_graph=True)
+    with pytest.raises(mgx.MgxError):
+        mgx.summa_run(A.ctypes.data, 4, A.ctypes.data, 4, dC, 4, m=4, n=4, k=4, panel=0)
+
+
+# ---- accuracy report -------------------------------------------------------------------
+
+def test_error_report_matches_oracle(mgx, orc):
+    m, n, k = 150, 130, 400
+    announce("error report vs oracle")
+    A = orc.fill(m, k, 1, seed=51)
+    B = orc.fill(k, n, 1, seed=52)
+    dA, dB = to_dev(A), to_dev(B)
+    dC = empty(m, n)
+    mgx.ABMultiply(dA, dB, dC, m, n, k, k, n, n, 32)
+    C = to_host(dC)
+    rows = np.array([0, 7, 149, 64])
+    rep = mgx.error_report(dA, dB, dC, m, n, k, k, n, n, rows)
+    hi, lo = orc.gemm_dd_rows(A, B, rows)
+    d = (C[rows] - hi) - lo
+    assert rep["rows_checked"] == 4
+    assert rep["normwise"] == pytest.approx(np.linalg.norm(d) / np.linalg.norm(hi), rel=1e-9)
+    assert rep["max_abs"] == pytest.approx(np.abs(d).max(), rel=1e-12)
+    absp = orc.gemm_abs_rows(A, B, rows)
+    assert rep["componentwise"] == pytest.approx((np.abs(d) / absp).max(), rel=1e-9)
+
+
+def test_error_report_planted_error(mgx, orc):
+    m, n, k = 64, 64, 64
+    announce("error report planted")
+    r = np.random.default_rng(2)
+    A = r.integers(-9, 9, size=(m, k)).astype(np.float64)
+    B = r.integers(-9, 9, size=(k, n)).astype(np.float64)
+    C = A @ B                                    # exact
+    C[5, 6] += 0.5
+    rep = mgx.error_report(to_dev(A), to_dev(B), to_dev(C), m, n, k, k, n, n, np.arange(m))
+    assert rep["max_abs"] == 0.5
+    C[5, 6] = np.nan
+    rep = mgx.error_report(to_dev(A), to_dev(B), to_dev(C), m, n, k, k, n, n, np.arange(m))
+    assert np.isinf(rep["normwise"])
