@@ -36,7 +36,7 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--slices", type=int, default=8)
+    ap.add_argument("--slices", type=int, default=7)
     ap.add_argument("--m", type=int, default=8192)
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--k", type=int, default=16384)
@@ -154,6 +154,15 @@ def main() -> int:
 
     # agreement of the two forms on the full share (both FP64-accurate)
     diff = float(torch.linalg.vector_norm(C - Cp) / torch.linalg.vector_norm(Cp))
+    # error of each form against a double-double A.B on evenly sampled rows
+    log("accuracy check (double-double, sampled rows)")
+    rows = list(range(0, m, max(1, m // 16)))[:16]
+    acc = {"rows_sampled": len(rows),
+           "split": mgx.error_report(A, B, C, m, n, k, k, n, n, rows),
+           "plain": mgx.error_report(A, B, Cp, m, n, k, k, n, n, rows)}
+    for form in ("split", "plain"):
+        acc[form].pop("rows_checked", None)
+    log(f"normwise error split {acc['split']['normwise']:.3e} plain {acc['plain']['normwise']:.3e}")
 
     tf_split = F / (ms_split * 1e-3) / 1e12
     tf_plain = F / (ms_plain * 1e-3) / 1e12
@@ -190,6 +199,8 @@ def main() -> int:
         "split_phase_ms": {"exponent_scan": round(scan_ms, 4), "slicing": round(slice_ms, 4),
                            "slice_pair_product": round(gemm_ms, 4)},
         "split_vs_plain_rel_fro": diff,
+        "accuracy_vs_double_double": acc,
+        "split_no_worse_than_plain": acc["split"]["normwise"] <= acc["plain"]["normwise"],
         "roofline": {
             "kernel": "ozaki_int8_gemm (fused slice-pair product)",
             "bound": "mfma",

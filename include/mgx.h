@@ -41,6 +41,13 @@ enum {
     MGX_FORM_BF16 = 2          /* Ozaki split: BF16 slices, exact FP32 MFMA accumulation */
 };
 
+/* Block-tile variants of the product kernels (autotuner knob). */
+enum {
+    MGX_TILE_DEFAULT = 0,   /* the large tile */
+    MGX_TILE_LARGE = 1,     /* FP64: 128x128 C block per workgroup; INT8: 256x256 */
+    MGX_TILE_SMALL = 2      /* FP64: 64x64; INT8: 128x128 */
+};
+
 /* Human-readable text of the last error on the calling thread. */
 const char *mgx_last_error(void);
 /* Library version string, e.g. "0.1.0". */
@@ -74,6 +81,13 @@ int mgx_dgemm_accumulate(const double *A, const double *B_chunk, double *C,
                          int64_t k_offset, int64_t k_chunk,
                          int64_t lda, int64_t ldb, int64_t ldc,
                          int w, void *stream);
+
+/* C = A.B (accumulate == 0) or C += A.B with an explicit tile variant.
+ * No counterpart in the reference (its tile is fixed at 32); used by the
+ * autotuner. */
+int mgx_dgemm_ex(int tile, int accumulate, const double *A, const double *B, double *C,
+                 int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc,
+                 void *stream);
 
 /* ---- constant-result validation ----------------------------------------
  * Replaces validate_matrix_C (1D-row-block/mulmat_kernel.cu:228-326).
@@ -123,6 +137,14 @@ int mgx_split_gemm_events(int form, int slices,
                           int64_t lda, int64_t ldb, int64_t ldc,
                           int accumulate, void *workspace, size_t workspace_bytes,
                           void *stream, void *const *events);
+
+/* mgx_split_gemm with an explicit tile variant (MGX_TILE_*). */
+int mgx_split_gemm_ex(int form, int slices, int tile,
+                      const double *A, const double *B, double *C,
+                      int64_t m, int64_t n, int64_t k,
+                      int64_t lda, int64_t ldb, int64_t ldc,
+                      int accumulate, void *workspace, size_t workspace_bytes,
+                      void *stream);
 
 /* Supported slice-count range for a split form (inclusive). */
 int mgx_split_slices_range(int form, int *min_slices, int *max_slices);
@@ -190,6 +212,48 @@ int mgx_summa_run(const mgx_summa_params *params,
 /* Pinned host memory for SUMMA strips (hipHostMalloc / hipHostFree). */
 int mgx_host_alloc(void **ptr, size_t bytes);
 int mgx_host_free(void *ptr);
+
+/* ---- autotuner measurement ------------------------------------------------
+ * One candidate configuration of the local product: form (MGX_FORM_FP64 or
+ * MGX_FORM_INT8), slices (INT8), tile (MGX_TILE_*) and SUMMA panel width.
+ * mgx_time_local_product runs C = sum over panels of A[:, k0:k0+kc] .
+ * B[k0:k0+kc, :] with device-resident operands (first panel overwrites C,
+ * later panels accumulate), `warmup` untimed then `reps` timed repetitions,
+ * and returns the mean milliseconds per full product (hipEvent timing).
+ * workspace may be NULL (library scratch) for the INT8 form. */
+typedef struct mgx_tune_config {
+    int form;
+    int slices;
+    int tile;
+    int64_t panel;
+} mgx_tune_config;
+
+int mgx_time_local_product(const mgx_tune_config *cfg,
+                           const double *A, const double *B, double *C,
+                           int64_t m, int64_t n, int64_t k,
+                           int64_t lda, int64_t ldb, int64_t ldc,
+                           void *workspace, size_t workspace_bytes,
+                           int warmup, int reps, double *ms_out);
+
+/* ---- learned cost model ---------------------------------------------------
+ * A 3-layer, width-64 FP32 MLP (in_dim -> 64 -> 64 -> 1, ReLU) trained and
+ * evaluated on the GPU by the library's own kernels.  It replaces the fitted
+ * runtime models of the reference autotuner (autotuning/predictive.py:100-233:
+ * power law, lstsq "physical" fit, ridge "ml" fit).  Parameters are one FP32
+ * vector: W1 [in_dim][64], b1 [64], W2 [64][64], b2 [64], W3 [64], b3.
+ * All arrays are host arrays; calls are synchronous. */
+#define MLP_MAX_IN 16
+#define MLP_HIDDEN 64
+int64_t mgx_mlp_param_count(int in_dim);                 /* -1 if in_dim is out of range */
+int mgx_mlp_init(float *theta, int in_dim, uint64_t seed);  /* He-uniform weights, zero biases */
+int mgx_mlp_predict(const float *theta, int in_dim, const float *X, int64_t n, float *y);
+/* Mean squared error over the n samples and its gradient. */
+int mgx_mlp_gradient(const float *theta, int in_dim, const float *X, const float *t,
+                     int64_t n, float *grad, float *loss);
+/* `epochs` full-batch Adam steps (beta 0.9 / 0.999, eps 1e-8, decoupled-free
+ * L2 `weight_decay` added to the gradient); theta is updated in place. */
+int mgx_mlp_train(float *theta, int in_dim, const float *X, const float *t, int64_t n,
+                  int epochs, float lr, float weight_decay, float *loss_history);
 
 /* ---- synthetic inputs ---------------------------------------------------
  * Fill a rows x cols row-major device matrix (ld) on `stream`.

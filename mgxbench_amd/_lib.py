@@ -46,6 +46,15 @@ class SummaStats(ctypes.Structure):
                 ("host_pinned", ctypes.c_int)]
 
 
+class TuneConfig(ctypes.Structure):
+    _fields_ = [("form", ctypes.c_int), ("slices", ctypes.c_int), ("tile", ctypes.c_int),
+                ("panel", ctypes.c_int64)]
+
+
+TILE_DEFAULT = 0
+TILE_LARGE = 1
+TILE_SMALL = 2
+
 _SIGNATURES = {
     "mgx_last_error": (ctypes.c_char_p, []),
     "mgx_version": (ctypes.c_char_p, []),
@@ -53,6 +62,12 @@ _SIGNATURES = {
     "mgx_dgemm": (_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _int, _vp]),
     "mgx_dgemm_accumulate": (_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64,
                                     _i64, _int, _vp]),
+    "mgx_dgemm_ex": (_int, [_int, _int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp]),
+    "mgx_split_gemm_ex": (_int, [_int, _int, _int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64,
+                                 _i64, _int, _vp, ctypes.c_size_t, _vp]),
+    "mgx_time_local_product": (_int, [ctypes.POINTER(TuneConfig), _vp, _vp, _vp, _i64, _i64, _i64,
+                                      _i64, _i64, _i64, _vp, ctypes.c_size_t, _int, _int,
+                                      ctypes.POINTER(_dbl)]),
     "mgx_validate_constant": (_int, [_vp, _i64, _i64, _dbl, _dbl, _dbl, _int,
                                      ctypes.POINTER(_int), ctypes.POINTER(_u64),
                                      ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]),
@@ -69,6 +84,12 @@ _SIGNATURES = {
                              ctypes.POINTER(SummaStats)]),
     "mgx_host_alloc": (_int, [ctypes.POINTER(_vp), ctypes.c_size_t]),
     "mgx_host_free": (_int, [_vp]),
+    "mgx_mlp_param_count": (_i64, [_int]),
+    "mgx_mlp_init": (_int, [_vp, _int, _u64]),
+    "mgx_mlp_predict": (_int, [_vp, _int, _vp, _i64, _vp]),
+    "mgx_mlp_gradient": (_int, [_vp, _int, _vp, _vp, _i64, _vp, _vp]),
+    "mgx_mlp_train": (_int, [_vp, _int, _vp, _vp, _i64, _int, ctypes.c_float, ctypes.c_float,
+                             _vp]),
     "mgx_fill": (_int, [_vp, _i64, _i64, _i64, _int, _u64, _dbl, _vp]),
     "mgx_fill_exponents": (_int, [_vp, _i64, _u64, _int, _int, _vp]),
     "mgx_scale_rows_cols": (_int, [_vp, _i64, _i64, _i64, _vp, _vp, _vp]),

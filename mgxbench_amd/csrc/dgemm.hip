@@ -9,8 +9,8 @@
 // and the accumulate form adds one per-chunk partial product into C.
 //
 // Design (gfx950): one workgroup of 256 lanes (4 wave64s) owns a 128x128
-// block of C; each wave owns a 64x64 sub-block held as 4x4 MFMA 16x16
-// accumulators (64 doubles per lane).  A and B k-slabs of depth 16 are
+// block of C (64x64 for the small-tile variant); each wave owns a quarter,
+// held as 16x16 MFMA accumulators (64 doubles per lane at 128x128).  A and B k-slabs of depth 16 are
 // staged through LDS with two buffers: the global loads of slab t+1 are in
 // flight while the MFMAs of slab t run, so one barrier per slab suffices.
 // A is stored k-major in LDS so that both MFMA operands are read as 16
@@ -22,12 +22,8 @@
 
 namespace {
 
-constexpr int BM = 128;
-constexpr int BN = 128;
 constexpr int BK = 16;
 constexpr int NT = 256;
-constexpr int SA = BM + 16;  // LDS row stride (doubles) of the k-major A slab
-constexpr int SB = BN + 16;  // LDS row stride (doubles) of the B slab
 constexpr int GROUP_M = 8;   // C tile rows per raster group
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -47,10 +43,18 @@ __device__ inline void tile_coords(int tiles_m, int tiles_n, int &tm, int &tn) {
     tn = r / gsize;
 }
 
-template <bool ACCUM>
-__global__ __launch_bounds__(NT, 2) void dgemm_mfma_128x128(
+// BM x BN block of C per workgroup, 2x2 waves of (BM/2) x (BN/2), each held as
+// (BM/32) x (BN/32) MFMA 16x16 accumulators.
+template <int BM, int BN, bool ACCUM>
+__global__ __launch_bounds__(NT, 2) void dgemm_mfma(
     const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C,
     int64_t M, int64_t N, int64_t K, int64_t lda, int64_t ldb, int64_t ldc) {
+    constexpr int SA = BM + 16;  // LDS row stride (doubles) of the k-major A slab
+    constexpr int SB = BN + 16;  // LDS row stride (doubles) of the B slab
+    constexpr int FI = BM / 32;  // MFMA tiles per wave along m
+    constexpr int FJ = BN / 32;  // MFMA tiles per wave along n
+    constexpr int EA = BM * BK / NT;  // A elements per lane per slab
+    constexpr int EB = BK * BN / NT;  // B elements per lane per slab
     __shared__ double As[2][BK][SA];
     __shared__ double Bs[2][BK][SB];
 
@@ -67,24 +71,23 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma_128x128(
     const int wm = wave >> 1;
     const int wn = wave & 1;
 
-    // Global->LDS load assignment: A slab is BM rows x BK cols, each lane
-    // takes 8 consecutive k of one row; B slab is BK rows x BN cols, each
-    // lane takes 8 consecutive columns of one row.
-    const int a_row = tid >> 1;
-    const int a_k = (tid & 1) * 8;
-    const int b_row = tid >> 4;
-    const int b_col = (tid & 15) * 8;
+    // Global->LDS load assignment: each lane takes EA consecutive k of one A
+    // row and EB consecutive columns of one B row.
+    const int a_row = tid / (BK / EA);
+    const int a_k = (tid % (BK / EA)) * EA;
+    const int b_row = tid / (BN / EB);
+    const int b_col = (tid % (BN / EB)) * EB;
 
     const int64_t ga_row = m0 + a_row;
     const bool a_row_ok = ga_row < M;
     const double *a_ptr = A + (a_row_ok ? ga_row : 0) * lda;
     const int64_t gb_col = n0 + b_col;
 
-    double ra[8], rb[8];
+    double ra[EA], rb[EB];
 
     auto load_slab = [&](int64_t k0) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
+        for (int j = 0; j < EA; ++j) {
             const int64_t kk = k0 + a_k + j;
             ra[j] = (a_row_ok && kk < K) ? a_ptr[kk] : 0.0;
         }
@@ -92,23 +95,23 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma_128x128(
         const bool b_row_ok = kb < K;
         const double *b_ptr = B + (b_row_ok ? kb : 0) * ldb;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
+        for (int j = 0; j < EB; ++j) {
             const int64_t c = gb_col + j;
             rb[j] = (b_row_ok && c < N) ? b_ptr[c] : 0.0;
         }
     };
     auto store_slab = [&](int buf) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) As[buf][a_k + j][a_row] = ra[j];
+        for (int j = 0; j < EA; ++j) As[buf][a_k + j][a_row] = ra[j];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) Bs[buf][b_row][b_col + j] = rb[j];
+        for (int j = 0; j < EB; ++j) Bs[buf][b_row][b_col + j] = rb[j];
     };
 
-    v4d acc[4][4];
+    v4d acc[FI][FJ];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < FI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < FJ; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
 
     const int nslabs = (int)((K + BK - 1) / BK);
     const int lk = lane >> 4;   // k row within an MFMA step
@@ -125,15 +128,15 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma_128x128(
         if (more) load_slab((int64_t)(s + 1) * BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
-            double af[4], bf[4];
+            double af[FI], bf[FJ];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) af[i] = As[buf][kk + lk][wm * 64 + i * 16 + lc];
+            for (int i = 0; i < FI; ++i) af[i] = As[buf][kk + lk][wm * (BM / 2) + i * 16 + lc];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bf[j] = Bs[buf][kk + lk][wn * 64 + j * 16 + lc];
+            for (int j = 0; j < FJ; ++j) bf[j] = Bs[buf][kk + lk][wn * (BN / 2) + j * 16 + lc];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < FI; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < FJ; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
         }
         if (more) store_slab(buf ^ 1);
@@ -143,15 +146,15 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma_128x128(
     // FP64 16x16 accumulator layout: lane l, register r holds row 4*r + l/16,
     // column l%16 of the fragment (register r covers rows 4r..4r+3).
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < FI; ++i) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int64_t row = m0 + wm * 64 + i * 16 + 4 * r + lk;
+            const int64_t row = m0 + wm * (BM / 2) + i * 16 + 4 * r + lk;
             if (row >= M) continue;
             double *c_row = C + row * ldc;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int64_t col = n0 + wn * 64 + j * 16 + lc;
+            for (int j = 0; j < FJ; ++j) {
+                const int64_t col = n0 + wn * (BN / 2) + j * 16 + lc;
                 if (col < N) {
                     if (ACCUM)
                         c_row[col] = c_row[col] + acc[i][j][r];
@@ -163,24 +166,40 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma_128x128(
     }
 }
 
-int launch_dgemm(bool accumulate, const double *A, const double *B, double *C, int64_t m,
-                 int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc, hipStream_t st) {
+template <int BM, int BN>
+int launch_tile(bool accumulate, const double *A, const double *B, double *C, int64_t m,
+                int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc, hipStream_t st) {
     const int64_t tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN);
     MGX_ARG_CHECK(tiles <= 0x7fffffffll, "matrix too large for one launch");
     dim3 grid((unsigned)tiles), block(NT);
     if (accumulate)
-        hipLaunchKernelGGL(dgemm_mfma_128x128<true>, grid, block, 0, st, A, B, C, m, n, k, lda, ldb,
-                           ldc);
-    else
-        hipLaunchKernelGGL(dgemm_mfma_128x128<false>, grid, block, 0, st, A, B, C, m, n, k, lda,
+        hipLaunchKernelGGL((dgemm_mfma<BM, BN, true>), grid, block, 0, st, A, B, C, m, n, k, lda,
                            ldb, ldc);
-    return mgx::launch_status("dgemm_mfma_128x128");
+    else
+        hipLaunchKernelGGL((dgemm_mfma<BM, BN, false>), grid, block, 0, st, A, B, C, m, n, k, lda,
+                           ldb, ldc);
+    return mgx::launch_status("dgemm_mfma");
+}
+
+int launch_dgemm(int tile, bool accumulate, const double *A, const double *B, double *C,
+                 int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc,
+                 hipStream_t st) {
+    if (tile == MGX_TILE_DEFAULT) tile = MGX_TILE_LARGE;
+    switch (tile) {
+        case MGX_TILE_LARGE:
+            return launch_tile<128, 128>(accumulate, A, B, C, m, n, k, lda, ldb, ldc, st);
+        case MGX_TILE_SMALL:
+            return launch_tile<64, 64>(accumulate, A, B, C, m, n, k, lda, ldb, ldc, st);
+        default:
+            return mgx::set_error(MGX_ERR_UNSUPPORTED, "unknown tile");
+    }
 }
 
 }  // namespace
 
-extern "C" int mgx_dgemm(const double *A, const double *B, double *C, int64_t m, int64_t n,
-                         int64_t k, int64_t lda, int64_t ldb, int64_t ldc, int w, void *stream) {
+static int dgemm_entry(int tile, const double *A, const double *B, double *C, int64_t m,
+                       int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc, int w,
+                       void *stream) {
     mgx::clear_error();
     MGX_ARG_CHECK(w > 0, "tile width w must be positive");
     MGX_ARG_CHECK(m >= 0 && n >= 0 && k >= 0, "negative matrix dimension");
@@ -191,13 +210,13 @@ extern "C" int mgx_dgemm(const double *A, const double *B, double *C, int64_t m,
         MGX_ARG_CHECK(A != nullptr && B != nullptr, "A or B is NULL");
         MGX_ARG_CHECK(lda >= k && ldb >= n, "lda < k or ldb < n");
     }
-    return launch_dgemm(false, A, B, C, m, n, k, lda, ldb, ldc, (hipStream_t)stream);
+    return launch_dgemm(tile, false, A, B, C, m, n, k, lda, ldb, ldc, (hipStream_t)stream);
 }
 
-extern "C" int mgx_dgemm_accumulate(const double *A, const double *B_chunk, double *C,
-                                    int64_t m, int64_t n, int64_t k_total, int64_t k_offset,
-                                    int64_t k_chunk, int64_t lda, int64_t ldb, int64_t ldc,
-                                    int w, void *stream) {
+static int dgemm_accumulate_entry(int tile, const double *A, const double *B_chunk, double *C,
+                                  int64_t m, int64_t n, int64_t k_total, int64_t k_offset,
+                                  int64_t k_chunk, int64_t lda, int64_t ldb, int64_t ldc, int w,
+                                  void *stream) {
     mgx::clear_error();
     MGX_ARG_CHECK(w > 0, "tile width w must be positive");
     MGX_ARG_CHECK(m >= 0 && n >= 0 && k_total >= 0 && k_offset >= 0 && k_chunk >= 0,
@@ -214,6 +233,27 @@ extern "C" int mgx_dgemm_accumulate(const double *A, const double *B_chunk, doub
         MGX_ARG_CHECK(lda >= k_offset + k_eff, "lda too small for k_offset + k_chunk");
         MGX_ARG_CHECK(ldb >= n, "ldb < n");
     }
-    return launch_dgemm(true, k_eff > 0 ? A + k_offset : A, B_chunk, C, m, n, k_eff, lda, ldb, ldc,
-                        (hipStream_t)stream);
+    return launch_dgemm(tile, true, k_eff > 0 ? A + k_offset : A, B_chunk, C, m, n, k_eff, lda,
+                        ldb, ldc, (hipStream_t)stream);
+}
+
+extern "C" int mgx_dgemm(const double *A, const double *B, double *C, int64_t m, int64_t n,
+                         int64_t k, int64_t lda, int64_t ldb, int64_t ldc, int w, void *stream) {
+    return dgemm_entry(MGX_TILE_DEFAULT, A, B, C, m, n, k, lda, ldb, ldc, w, stream);
+}
+
+extern "C" int mgx_dgemm_accumulate(const double *A, const double *B_chunk, double *C,
+                                    int64_t m, int64_t n, int64_t k_total, int64_t k_offset,
+                                    int64_t k_chunk, int64_t lda, int64_t ldb, int64_t ldc,
+                                    int w, void *stream) {
+    return dgemm_accumulate_entry(MGX_TILE_DEFAULT, A, B_chunk, C, m, n, k_total, k_offset,
+                                  k_chunk, lda, ldb, ldc, w, stream);
+}
+
+extern "C" int mgx_dgemm_ex(int tile, int accumulate, const double *A, const double *B, double *C,
+                            int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc,
+                            void *stream) {
+    if (accumulate)
+        return dgemm_accumulate_entry(tile, A, B, C, m, n, k, 0, k, lda, ldb, ldc, 32, stream);
+    return dgemm_entry(tile, A, B, C, m, n, k, lda, ldb, ldc, 32, stream);
 }

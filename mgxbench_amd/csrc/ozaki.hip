@@ -5,23 +5,26 @@
 // ABMultiply / ABMultiplyAccumulateAsync (1D-row-block/mulmat_kernel.cu:86,
 // 148) through a different arithmetic.
 //
-// Algorithm (INT8 form, BITS = 7 bits per slice):
+// Algorithm (INT8 form):
 //  1. e_i = frexp exponent of max_k |A_ik| (so |A_ik| < 2^e_i), f_j likewise
 //     for column j of B.  A row/column holding Inf/NaN gets a sentinel and
 //     its outputs become NaN.
-//  2. x = A_ik * 2^-e_i in (-1,1).  Slice p (1..s) is
-//        sigma_p = trunc(x * 2^(7p)) - 128 * trunc(x * 2^(7(p-1)))
-//     an integer in [-127,127]; sum_p sigma_p 2^(-7p) is x truncated to 7s
-//     fractional bits.  Every step is exact in FP64.  B is sliced per column
-//     with f_j and stored transposed (n-major, k contiguous).
-//  3. For each weight group t = 2..s+1 (slice pairs p+q = t), the INT8 MFMA
-//     accumulates G_t = sum_{p+q=t} sum_k sigmaA_p sigmaB_q exactly in INT32.
-//     The K range is cut into chunks of `chunk` 64-deep tiles (whole LDS
-//     stages), chosen so that pairs(t) * chunk * 64 * 127^2 < 2^31.
+//  2. x = A_ik * 2^-e_i in (-1,1) is cut into s signed digits by round to
+//     nearest: sigma_1 = rint(x 2^6), then sigma_p = rint(256 r) of the
+//     previous remainder r, with a carry pass that keeps every digit in
+//     [-128, 127] (slice_digits).  Place values are 2^(2-8p); the residual
+//     after s slices is at most 2^(1-8s).  Every step is exact in FP64.  B is
+//     sliced per column with f_j and stored transposed (n-major, k contiguous).
+//  3. For each weight group t = 2..s+1 (slice pairs p+q = t, place value
+//     2^(4-8t)), the INT8 MFMA accumulates G_t = sum_{p+q=t} sum_k sigmaA_p
+//     sigmaB_q exactly in INT32.  The K range is cut into chunks of `chunk`
+//     64-deep tiles (whole LDS stages), chosen so that
+//     pairs(t) * chunk * 64 * 128^2 < 2^31.
 //  4. After each chunk the sums are folded into C in FP64:
-//     x = ldexp((double)G, e_i + f_j - 7t) (exact); the first fold writes
+//     x = ldexp((double)G, e_i + f_j + 4 - 8t) (exact); the first fold writes
 //     C = x (or C_in + x for the accumulate form), later folds C = C + x.
-// Pairs with p+q > s+1 are dropped; their weight is below 2^(-7s) relative.
+// Pairs with p+q > s+1 are dropped; with the residuals they bound the error
+// by about (s+2) 2^(2-8s) K 2^(e_i+f_j).
 #include "mgx_internal.hpp"
 
 #include <algorithm>
@@ -33,7 +36,7 @@ constexpr int K_ALIGN = 256;       // slice rows are zero-padded to this multipl
 constexpr int EXP_BAD = 0x40000000;
 constexpr int MIN_SLICES = 1;
 constexpr int MAX_SLICES = 9;
-constexpr long long SLICE_MAX_PRODUCT = 127ll * 127ll;
+constexpr long long SLICE_MAX_PRODUCT = 128ll * 128ll;  // largest |sigma_p sigma_q|
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
@@ -108,30 +111,51 @@ __global__ void exponents_kernel(const unsigned long long *maxbits, int32_t *e, 
 
 // ---- slice extraction -----------------------------------------------------
 
+// Slice digits of one scaled value x in (-1,1), most significant first:
+//   y = x * 2^6, sigma_1 = rint(y), r = y - sigma_1            (|sigma_1| <= 64)
+//   y = 256 r,   sigma_p = rint(y), r = y - sigma_p  (p >= 2)  (|sigma_p| <= 128)
+// then a carry pass from the least significant digit up turns every 128 into
+// -128 and adds 1 to the digit above (|sigma_1| <= 65 afterwards), so all
+// digits fit int8.  Every step is exact in FP64, and
+//   x = sum_p sigma_p 2^(2-8p) + r_s 2^(2-8s),  |r_s| <= 1/2.
+__device__ inline void slice_digits(double x, int s, int *dig) {
+    double y = x * 64.0;
+    for (int p = 0; p < s; ++p) {
+        const double d = rint(y);
+        dig[p] = (int)d;
+        y = (y - d) * 256.0;
+    }
+    for (int p = s - 1; p > 0; --p) {
+        if (dig[p] >= 128) {
+            dig[p] -= 256;
+            dig[p - 1] += 1;
+        }
+    }
+}
+
 // Writes the s slices of 16 consecutive scaled values x[0..15] as 16 bytes
 // each.  dst points at slice 1; consecutive slices are `slice_stride` apart.
 __device__ inline void emit_slices16(const double *x, int s, int8_t *dst, int64_t slice_stride) {
-    double prev[16];
+    unsigned int words[MAX_SLICES][4];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) prev[i] = 0.0;
-    double scale = 1.0;
-    for (int p = 1; p <= s; ++p) {
-        scale *= 128.0;
-        v4i packed;
+    for (int p = 0; p < MAX_SLICES; ++p)
 #pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            unsigned int word = 0;
+        for (int w = 0; w < 4; ++w) words[p][w] = 0u;
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int i = w * 4 + b;
-                const double tp = trunc(x[i] * scale);
-                const int sig = (int)(tp - 128.0 * prev[i]);
-                prev[i] = tp;
-                word |= ((unsigned int)(sig & 0xff)) << (8 * b);
-            }
-            packed[w] = (int)word;
+    for (int i = 0; i < 16; ++i) {
+        int dig[MAX_SLICES];
+        slice_digits(x[i], s, dig);
+#pragma unroll
+        for (int p = 0; p < MAX_SLICES; ++p)
+            if (p < s) words[p][i / 4] |= ((unsigned int)(dig[p] & 0xff)) << (8 * (i % 4));
+    }
+#pragma unroll
+    for (int p = 0; p < MAX_SLICES; ++p) {
+        if (p < s) {
+            const v4i packed = {(int)words[p][0], (int)words[p][1], (int)words[p][2],
+                                (int)words[p][3]};
+            *(v4i *)(dst + (int64_t)p * slice_stride) = packed;
         }
-        *(v4i *)(dst + (int64_t)(p - 1) * slice_stride) = packed;
     }
 }
 
@@ -195,13 +219,10 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double *__restric
 //     C = C + x             on every later fold.
 // so C is read and written once per fold; C stays in HBM between folds.
 
-constexpr int TB = 256;         // C block edge handled by one workgroup
 constexpr int TBK = 128;        // k bytes per LDS stage
 constexpr int TSTR = TBK + 16;  // LDS row stride in bytes
 constexpr int TNT = 256;        // lanes per workgroup
 constexpr int TGROUP_M = 8;     // C block rows per raster group
-constexpr int FOLD_STR = 128 + 4;  // ints per row of a wave's fold scratch
-static_assert(4 * 32 * FOLD_STR * 4 <= 2 * TB * TSTR, "fold scratch must fit one LDS stage");
 
 // Walks the (group t, K chunk, pair p, k tile) steps in execution order.
 struct StepIter {
@@ -247,10 +268,16 @@ struct StepIter {
     }
 };
 
-__global__ __launch_bounds__(TNT, 1) void ozaki_int8_gemm_256x256(
+// TB = 256 (large tile) or 128 (small tile); waves own (TB/2) x (TB/2).
+template <int TB>
+__global__ __launch_bounds__(TNT, TB == 256 ? 1 : 2) void ozaki_int8_gemm(
     const int8_t *__restrict__ Asl, const int8_t *__restrict__ Bsl,
     const int32_t *__restrict__ rexp, const int32_t *__restrict__ cexp, double *__restrict__ C,
     int64_t M, int64_t N, int64_t Kp, int64_t ldc, int s, int64_t nsteps, int accumulate) {
+    constexpr int WT = TB / 2;             // wave tile edge
+    constexpr int FI = WT / 16;            // 16x16 fragments per wave edge
+    constexpr int FOLD_STR = WT + 4;       // ints per row of a wave's fold scratch
+    static_assert(4 * 32 * FOLD_STR * 4 <= 2 * TB * TSTR, "fold scratch must fit one LDS stage");
     // One contiguous LDS stage per buffer: A tile at offset 0, B tile after it.
     __shared__ __attribute__((aligned(16))) int8_t lds[2][2 * TB * TSTR];
 
@@ -280,7 +307,7 @@ __global__ __launch_bounds__(TNT, 1) void ozaki_int8_gemm_256x256(
     const int lk = lane >> 4;
     const int lc = lane & 15;
 
-    // Load assignment: 16-byte vector v = tid + 256*i (i < 8) covers row
+    // Load assignment: 16-byte vector v = tid + 256*i (i < NV) covers row
     // v / 8, byte offset (v % 8) * 16 of the BK = 128 byte stage.
     constexpr int VPR = TBK / 16;              // vectors per row
     constexpr int NV = TB * VPR / TNT;         // vectors per lane per operand
@@ -315,11 +342,11 @@ __global__ __launch_bounds__(TNT, 1) void ozaki_int8_gemm_256x256(
         }
     };
 
-    v4i acc[8][8];
+    v4i acc[FI][FI];
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < FI; ++i)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) acc[i][j] = vzero;
+        for (int j = 0; j < FI; ++j) acc[i][j] = vzero;
 
     const int ktiles = (int)(Kp / KT);
     StepIter it;
@@ -338,18 +365,18 @@ __global__ __launch_bounds__(TNT, 1) void ozaki_int8_gemm_256x256(
 
 #pragma unroll 1
         for (int kk = 0; kk < TBK / KT; ++kk) {
-            v4i af[8], bf[8];
+            v4i af[FI], bf[FI];
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
-                af[i] = *(const v4i *)&lds[buf][(wm * 128 + i * 16 + lc) * TSTR + kk * KT + 16 * lk];
+            for (int i = 0; i < FI; ++i)
+                af[i] = *(const v4i *)&lds[buf][(wm * WT + i * 16 + lc) * TSTR + kk * KT + 16 * lk];
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-                bf[j] = *(const v4i *)&lds[buf][TB * TSTR + (wn * 128 + j * 16 + lc) * TSTR + kk * KT +
+            for (int j = 0; j < FI; ++j)
+                bf[j] = *(const v4i *)&lds[buf][TB * TSTR + (wn * WT + j * 16 + lc) * TSTR + kk * KT +
                                                 16 * lk];
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
+            for (int i = 0; i < FI; ++i)
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
+                for (int j = 0; j < FI; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
         }
 
@@ -359,23 +386,27 @@ __global__ __launch_bounds__(TNT, 1) void ozaki_int8_gemm_256x256(
 
         if (fold) {
             // Fold through LDS: the buffer read by this step is free once every
-            // wave has passed the barrier.  Four passes; in pass q each wave
+            // wave has passed the barrier.  FI/2 passes; in pass q each wave
             // dumps its INT32 accumulators of fragment rows 2q, 2q+1 (32 rows
-            // x 128 columns, rows padded to FOLD_STR ints) and then folds them
-            // into C row by row, two adjacent columns per lane.
+            // x WT columns, rows padded to FOLD_STR ints) and then folds them
+            // into C row by row, WT/64 adjacent columns per lane.
+            constexpr int CPL = WT / 64;   // columns per lane in the fold
             __syncthreads();
             int *fold_buf = (int *)&lds[buf][0] + wave * (32 * FOLD_STR);
-            const int64_t fcol = n0 + wn * 128 + 2 * lane;
-            int ec0 = 0, ec1 = 0;
-            if (fcol < N) ec0 = cexp[fcol];
-            if (fcol + 1 < N) ec1 = cexp[fcol + 1];
+            int64_t fcol[CPL];
+            int ecol[CPL];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int c = 0; c < CPL; ++c) {
+                fcol[c] = n0 + wn * WT + CPL * lane + c;
+                ecol[c] = fcol[c] < N ? cexp[fcol[c]] : 0;
+            }
+#pragma unroll
+            for (int q = 0; q < FI / 2; ++q) {
                 // INT32 layout: lane l, register r holds row 4*(l/16)+r, column l%16.
 #pragma unroll
                 for (int ii = 0; ii < 2; ++ii)
 #pragma unroll
-                    for (int j = 0; j < 8; ++j)
+                    for (int j = 0; j < FI; ++j)
 #pragma unroll
                         for (int r = 0; r < 4; ++r)
                             fold_buf[(ii * 16 + 4 * lk + r) * FOLD_STR + j * 16 + lc] =
@@ -383,31 +414,27 @@ __global__ __launch_bounds__(TNT, 1) void ozaki_int8_gemm_256x256(
                 __syncthreads();
 #pragma unroll 2
                 for (int rho = 0; rho < 32; ++rho) {
-                    const int64_t row = m0 + wm * 128 + 32 * q + rho;
+                    const int64_t row = m0 + wm * WT + 32 * q + rho;
                     if (row >= M) break;
                     const int er = rexp[row];
-                    const int g0 = fold_buf[rho * FOLD_STR + 2 * lane];
-                    const int g1 = fold_buf[rho * FOLD_STR + 2 * lane + 1];
                     double *c_row = C + row * ldc;
-                    if (fcol < N) {
-                        const double x = (er == EXP_BAD || ec0 == EXP_BAD)
-                                             ? __longlong_as_double(0x7ff8000000000000ll)
-                                             : ldexp((double)g0, er + ec0 - 7 * t_cur);
-                        c_row[fcol] = (first_fold && !accumulate) ? x : c_row[fcol] + x;
-                    }
-                    if (fcol + 1 < N) {
-                        const double x = (er == EXP_BAD || ec1 == EXP_BAD)
-                                             ? __longlong_as_double(0x7ff8000000000000ll)
-                                             : ldexp((double)g1, er + ec1 - 7 * t_cur);
-                        c_row[fcol + 1] = (first_fold && !accumulate) ? x : c_row[fcol + 1] + x;
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) {
+                        if (fcol[c] < N) {
+                            const int g = fold_buf[rho * FOLD_STR + CPL * lane + c];
+                            const double x = (er == EXP_BAD || ecol[c] == EXP_BAD)
+                                                 ? __longlong_as_double(0x7ff8000000000000ll)
+                                                 : ldexp((double)g, er + ecol[c] + 4 - 8 * t_cur);
+                            c_row[fcol[c]] = (first_fold && !accumulate) ? x : c_row[fcol[c]] + x;
+                        }
                     }
                 }
                 __syncthreads();
             }
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
+            for (int i = 0; i < FI; ++i)
 #pragma unroll
-                for (int j = 0; j < 8; ++j) acc[i][j] = vzero;
+                for (int j = 0; j < FI; ++j) acc[i][j] = vzero;
             first_fold = false;
         }
         __syncthreads();
@@ -473,11 +500,15 @@ extern "C" size_t mgx_split_workspace_bytes(int form, int slices, int64_t m, int
     return split_layout(slices, m, n, k).total;
 }
 
-static int split_gemm_impl(int form, int slices, const double *A, const double *B, double *C,
-                           int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc,
-                           int accumulate, void *workspace, size_t workspace_bytes, void *stream,
-                           void *const *events) {
+static int split_gemm_impl(int form, int slices, int tile, const double *A, const double *B,
+                           double *C, int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb,
+                           int64_t ldc, int accumulate, void *workspace, size_t workspace_bytes,
+                           void *stream, void *const *events) {
     mgx::clear_error();
+    if (tile == MGX_TILE_DEFAULT) tile = MGX_TILE_LARGE;
+    if (tile != MGX_TILE_LARGE && tile != MGX_TILE_SMALL)
+        return mgx::set_error(MGX_ERR_UNSUPPORTED, "unknown tile");
+    const int64_t TBsz = tile == MGX_TILE_LARGE ? 256 : 128;
     if (form != MGX_FORM_INT8)
         return mgx::set_error(MGX_ERR_UNSUPPORTED, "split form not supported");
     if (slices < MIN_SLICES || slices > MAX_SLICES)
@@ -495,7 +526,7 @@ static int split_gemm_impl(int form, int slices, const double *A, const double *
     MGX_ARG_CHECK(A != nullptr && B != nullptr, "A or B is NULL");
     MGX_ARG_CHECK(lda >= k && ldb >= n, "lda < k or ldb < n");
     const SplitLayout L = split_layout(slices, m, n, k);
-    const int64_t tiles = ((m + TB - 1) / TB) * ((n + TB - 1) / TB);
+    const int64_t tiles = ((m + TBsz - 1) / TBsz) * ((n + TBsz - 1) / TBsz);
     MGX_ARG_CHECK(tiles <= 0x7fffffffll, "matrix too large for one launch");
 
     char *ws = (char *)workspace;
@@ -556,9 +587,13 @@ static int split_gemm_impl(int form, int slices, const double *A, const double *
     }
     if ((rc = mark(2)) != MGX_OK) return rc;
     const int64_t nsteps = (L.Kp / TBK) * (int64_t)(slices * (slices + 1) / 2);
-    hipLaunchKernelGGL(ozaki_int8_gemm_256x256, dim3((unsigned)tiles), dim3(TNT), 0, st, Asl, Bsl,
-                       rexp, cexp, C, m, n, L.Kp, ldc, slices, nsteps, accumulate ? 1 : 0);
-    if ((rc = mgx::launch_status("ozaki_int8_gemm_256x256")) != MGX_OK) return rc;
+    if (tile == MGX_TILE_LARGE)
+        hipLaunchKernelGGL(ozaki_int8_gemm<256>, dim3((unsigned)tiles), dim3(TNT), 0, st, Asl, Bsl,
+                           rexp, cexp, C, m, n, L.Kp, ldc, slices, nsteps, accumulate ? 1 : 0);
+    else
+        hipLaunchKernelGGL(ozaki_int8_gemm<128>, dim3((unsigned)tiles), dim3(TNT), 0, st, Asl, Bsl,
+                           rexp, cexp, C, m, n, L.Kp, ldc, slices, nsteps, accumulate ? 1 : 0);
+    if ((rc = mgx::launch_status("ozaki_int8_gemm")) != MGX_OK) return rc;
     return mark(3);
 }
 
@@ -566,14 +601,22 @@ extern "C" int mgx_split_gemm(int form, int slices, const double *A, const doubl
                               int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb,
                               int64_t ldc, int accumulate, void *workspace,
                               size_t workspace_bytes, void *stream) {
-    return split_gemm_impl(form, slices, A, B, C, m, n, k, lda, ldb, ldc, accumulate, workspace,
-                           workspace_bytes, stream, nullptr);
+    return split_gemm_impl(form, slices, MGX_TILE_DEFAULT, A, B, C, m, n, k, lda, ldb, ldc,
+                           accumulate, workspace, workspace_bytes, stream, nullptr);
 }
 
 extern "C" int mgx_split_gemm_events(int form, int slices, const double *A, const double *B,
                                      double *C, int64_t m, int64_t n, int64_t k, int64_t lda,
                                      int64_t ldb, int64_t ldc, int accumulate, void *workspace,
                                      size_t workspace_bytes, void *stream, void *const *events) {
-    return split_gemm_impl(form, slices, A, B, C, m, n, k, lda, ldb, ldc, accumulate, workspace,
-                           workspace_bytes, stream, events);
+    return split_gemm_impl(form, slices, MGX_TILE_DEFAULT, A, B, C, m, n, k, lda, ldb, ldc,
+                           accumulate, workspace, workspace_bytes, stream, events);
+}
+
+extern "C" int mgx_split_gemm_ex(int form, int slices, int tile, const double *A, const double *B,
+                                 double *C, int64_t m, int64_t n, int64_t k, int64_t lda,
+                                 int64_t ldb, int64_t ldc, int accumulate, void *workspace,
+                                 size_t workspace_bytes, void *stream) {
+    return split_gemm_impl(form, slices, tile, A, B, C, m, n, k, lda, ldb, ldc, accumulate,
+                           workspace, workspace_bytes, stream, nullptr);
 }

@@ -43,6 +43,9 @@ _SIG = {
     "orc_split_chunk_tiles": (_int, [_int, _int]),
     "orc_split_gemm_int8": (_int, [_int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64,
                                    _int, _int]),
+    "orc_mlp_param_count": (_i64, [_int]),
+    "orc_mlp_forward": (None, [_vp, _int, _vp, _i64, _vp]),
+    "orc_mlp_gradient": (_dbl, [_vp, _int, _vp, _vp, _i64, _vp]),
 }
 
 _lib = None
@@ -202,3 +205,25 @@ def split_gemm_int8(A: np.ndarray, B: np.ndarray, s: int, C: np.ndarray | None =
     if rc != 0:
         raise RuntimeError(f"orc_split_gemm_int8 failed: {rc}")
     return out
+
+
+def mlp_param_count(D: int) -> int:
+    return int(lib().orc_mlp_param_count(D))
+
+
+def mlp_forward(theta: np.ndarray, X: np.ndarray) -> np.ndarray:
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    y = np.empty(X.shape[0])
+    lib().orc_mlp_forward(_p(th), X.shape[1], _p(X), X.shape[0], _p(y))
+    return y
+
+
+def mlp_gradient(theta: np.ndarray, X: np.ndarray, t: np.ndarray):
+    """(loss, gradient) of the mean squared error, in double precision."""
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    g = np.empty(mlp_param_count(X.shape[1]))
+    loss = lib().orc_mlp_gradient(_p(th), X.shape[1], _p(X), _p(t), X.shape[0], _p(g))
+    return float(loss), g

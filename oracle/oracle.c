@@ -263,7 +263,7 @@ ORC_API int64_t orc_validate_2d(const double *C, int64_t count, double expected)
     return -1;
 }
 
-/* ---- split-precision model (INT8 slices, 7 bits each) --------------------- */
+/* ---- split-precision model (INT8 slices, rounded signed digits) ------------ */
 
 #define ORC_EXP_BAD 0x40000000
 #define ORC_KT 64
@@ -303,15 +303,24 @@ ORC_API void orc_split_exponents(const double *A, const double *B, int64_t m, in
     }
 }
 
-/* Slices of one scaled value x in (-1,1): sig[p-1], p = 1..s. */
+/* Digits of one scaled value x in (-1,1): sig[p-1], p = 1..s, place value
+ * 2^(2-8p).  sigma_1 = rint(x 2^6); sigma_p = rint(256 r) of the previous
+ * remainder; then carries from the least significant digit turn 128 into
+ * -128 (+1 on the digit above). */
 ORC_API void orc_split_value(double x, int s, int8_t *sig) {
-    double prev = 0.0, scale = 1.0;
-    for (int p = 1; p <= s; ++p) {
-        scale *= 128.0;
-        const double tp = trunc(x * scale);
-        sig[p - 1] = (int8_t)(int)(tp - 128.0 * prev);
-        prev = tp;
+    int dig[16];
+    double y = x * 64.0;
+    for (int p = 0; p < s; ++p) {
+        const double d = rint(y);
+        dig[p] = (int)d;
+        y = (y - d) * 256.0;
     }
+    for (int p = s - 1; p > 0; --p)
+        if (dig[p] >= 128) {
+            dig[p] -= 256;
+            dig[p - 1] += 1;
+        }
+    for (int p = 0; p < s; ++p) sig[p] = (int8_t)dig[p];
 }
 
 /* Group-t chunk length in 64-deep tiles, the INT32 bound of the product kernel. */
@@ -319,7 +328,7 @@ ORC_API int orc_split_chunk_tiles(int s, int t) {
     const int pmin = t - s > 1 ? t - s : 1;
     const int pmax = t - 1 < s ? t - 1 : s;
     const long long np = pmax - pmin + 1;
-    long long c = 2147483647ll / (np * 127ll * 127ll * ORC_KT);
+    long long c = 2147483647ll / (np * 128ll * 128ll * ORC_KT);
     /* whole 128-byte LDS stages of the product kernel */
     c = (c / 2 < 1 ? 1 : c / 2) * 2;
     return (int)c;
@@ -349,14 +358,14 @@ ORC_API int orc_split_gemm_int8(int s, const double *A, const double *B, double 
     orc_split_exponents(A, B, m, n, k, lda, ldb, rexp, cexp);
     for (int64_t i = 0; i < m; ++i)
         for (int64_t kk = 0; kk < k; ++kk) {
-            int8_t sig[9];
+            int8_t sig[16];
             const double x = rexp[i] == ORC_EXP_BAD ? 0.0 : ldexp(A[i * lda + kk], -rexp[i]);
             orc_split_value(x, s, sig);
             for (int p = 0; p < s; ++p) As[((size_t)p * m + i) * k + kk] = sig[p];
         }
     for (int64_t j = 0; j < n; ++j)
         for (int64_t kk = 0; kk < k; ++kk) {
-            int8_t sig[9];
+            int8_t sig[16];
             const double x = cexp[j] == ORC_EXP_BAD ? 0.0 : ldexp(B[kk * ldb + j], -cexp[j]);
             orc_split_value(x, s, sig);
             for (int p = 0; p < s; ++p) Bs[((size_t)p * n + j) * k + kk] = sig[p];
@@ -386,7 +395,7 @@ ORC_API int orc_split_gemm_int8(int s, const double *A, const double *B, double 
                         for (int64_t kk = k0; kk < k1; ++kk) g += (long long)ar[kk] * br[kk];
                     }
                     if (g > 2147483647ll || g < -2147483648ll) overflow = 1;
-                    const double x = bad ? NAN : ldexp((double)g, rexp[i] + cexp[j] - 7 * t);
+                    const double x = bad ? NAN : ldexp((double)g, rexp[i] + cexp[j] + 4 - 8 * t);
                     if (first)
                         v = accumulate ? v + x : x;
                     else
@@ -399,4 +408,86 @@ ORC_API int orc_split_gemm_int8(int s, const double *A, const double *B, double 
     }
     free(rexp); free(cexp); free(As); free(Bs);
     return overflow ? -3 : 0;
+}
+
+/* ---- cost-model MLP (double precision) ------------------------------------ */
+/* Same network and parameter layout as libmgx's FP32 MLP: W1 [D][64], b1,
+ * W2 [64][64], b2, W3 [64], b3; loss = mean (y - t)^2.  Evaluated in double
+ * so the GPU's FP32 gradients can be checked against it; pinned itself by
+ * central finite differences. */
+#define ORC_H 64
+static void mlp_offsets(int D, int64_t *w1, int64_t *b1, int64_t *w2, int64_t *b2, int64_t *w3,
+                        int64_t *b3) {
+    *w1 = 0;
+    *b1 = (int64_t)D * ORC_H;
+    *w2 = *b1 + ORC_H;
+    *b2 = *w2 + ORC_H * ORC_H;
+    *w3 = *b2 + ORC_H;
+    *b3 = *w3 + ORC_H;
+}
+
+ORC_API int64_t orc_mlp_param_count(int D) { return (int64_t)D * ORC_H + ORC_H * ORC_H + 3 * ORC_H + 1; }
+
+ORC_API void orc_mlp_forward(const double *th, int D, const double *X, int64_t n, double *y) {
+    int64_t w1, b1, w2, b2, w3, b3;
+    mlp_offsets(D, &w1, &b1, &w2, &b2, &w3, &b3);
+    for (int64_t s = 0; s < n; ++s) {
+        double h1[ORC_H], out = th[b3];
+        for (int j = 0; j < ORC_H; ++j) {
+            double a = th[b1 + j];
+            for (int d = 0; d < D; ++d) a += X[s * D + d] * th[w1 + d * ORC_H + j];
+            h1[j] = a > 0.0 ? a : 0.0;
+        }
+        for (int j = 0; j < ORC_H; ++j) {
+            double a = th[b2 + j];
+            for (int i = 0; i < ORC_H; ++i) a += h1[i] * th[w2 + i * ORC_H + j];
+            out += (a > 0.0 ? a : 0.0) * th[w3 + j];
+        }
+        y[s] = out;
+    }
+}
+
+/* Loss (returned) and gradient (g, param_count entries) of the mean squared error. */
+ORC_API double orc_mlp_gradient(const double *th, int D, const double *X, const double *t,
+                                int64_t n, double *g) {
+    int64_t w1, b1, w2, b2, w3, b3;
+    mlp_offsets(D, &w1, &b1, &w2, &b2, &w3, &b3);
+    const int64_t P = orc_mlp_param_count(D);
+    for (int64_t p = 0; p < P; ++p) g[p] = 0.0;
+    double loss = 0.0;
+    for (int64_t s = 0; s < n; ++s) {
+        double a1[ORC_H], h1[ORC_H], a2[ORC_H], h2[ORC_H], y = th[b3];
+        for (int j = 0; j < ORC_H; ++j) {
+            double a = th[b1 + j];
+            for (int d = 0; d < D; ++d) a += X[s * D + d] * th[w1 + d * ORC_H + j];
+            a1[j] = a;
+            h1[j] = a > 0.0 ? a : 0.0;
+        }
+        for (int j = 0; j < ORC_H; ++j) {
+            double a = th[b2 + j];
+            for (int i = 0; i < ORC_H; ++i) a += h1[i] * th[w2 + i * ORC_H + j];
+            a2[j] = a;
+            h2[j] = a > 0.0 ? a : 0.0;
+            y += h2[j] * th[w3 + j];
+        }
+        const double diff = y - t[s];
+        loss += diff * diff / (double)n;
+        const double dy = 2.0 * diff / (double)n;
+        g[b3] += dy;
+        double dh2[ORC_H];
+        for (int j = 0; j < ORC_H; ++j) {
+            g[w3 + j] += h2[j] * dy;
+            dh2[j] = a2[j] > 0.0 ? th[w3 + j] * dy : 0.0;
+            g[b2 + j] += dh2[j];
+            for (int i = 0; i < ORC_H; ++i) g[w2 + i * ORC_H + j] += h1[i] * dh2[j];
+        }
+        for (int i = 0; i < ORC_H; ++i) {
+            double back = 0.0;
+            for (int k = 0; k < ORC_H; ++k) back += th[w2 + i * ORC_H + k] * dh2[k];
+            const double dh1 = a1[i] > 0.0 ? back : 0.0;
+            g[b1 + i] += dh1;
+            for (int d = 0; d < D; ++d) g[w1 + d * ORC_H + i] += X[s * D + d] * dh1;
+        }
+    }
+    return loss;
 }

@@ -2,8 +2,8 @@
 
 Bar: bit-exact against the oracle's model of the split product (integer
 slice sums are exact and the FP64 reconstruction is one fixed sequence of
-fma / ldexp / add), plus accuracy against the double-double oracle: at s = 8
-the normwise error must be no worse than the plain FP64 kernel's on the same
+ldexp / add), plus accuracy against the double-double oracle: at s = 7 the
+normwise error must be no worse than the plain FP64 kernel's on the same
 inputs.
 """
 import numpy as np
@@ -65,7 +65,7 @@ def test_exact_on_7bit_integers(mgx, orc):
     A = r.integers(-127, 128, size=(m, k)).astype(np.float64)
     B = r.integers(-127, 128, size=(k, n)).astype(np.float64)
     exact = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float64)
-    for s in (1, 8):
+    for s in (3, 8):
         assert np.array_equal(_split(mgx, A, B, s), exact)
 
 
@@ -103,7 +103,7 @@ def test_accuracy_n1024_no_worse_than_plain(mgx, orc):
     dC = empty(n, n)
     mgx.ABMultiply(dA, dB, dC, n, n, n, n, n, n, 32)
     plain = to_host(dC)
-    mgx.split_gemm(dA, dB, dC, n, n, n, n, n, n, slices=8)
+    mgx.split_gemm(dA, dB, dC, n, n, n, n, n, n, slices=7)
     split = to_host(dC)
     hi, lo = orc.gemm_dd_rows(A, B)
     e_plain = rel_fro(plain, hi)

@@ -177,27 +177,49 @@ def test_validate_2d_rule(orc):
 # ---- split-precision model -------------------------------------------------------------
 
 def _py_slices(x: float, s: int):
-    q = Fraction(x)
-    out, prev = [], 0
-    for p in range(1, s + 1):
-        tp = int(q * 2 ** (7 * p))          # int() truncates toward zero
-        out.append(tp - 128 * prev)
-        prev = tp
-    return out
+    """Exact digits: rint (half to even) of x 2^6, then of 256 * remainder,
+    then carries 128 -> -128 (+1 above) from the least significant digit."""
+    y = Fraction(x) * 64
+    dig = []
+    for _ in range(s):
+        d = round(y)                       # Fraction round: half to even
+        dig.append(d)
+        y = (y - d) * 256
+    for p in range(s - 1, 0, -1):
+        if dig[p] >= 128:
+            dig[p] -= 256
+            dig[p - 1] += 1
+    return dig
 
 
 @pytest.mark.parametrize("s", [1, 3, 7, 8, 9])
 def test_split_value_slices(orc, s):
     r = np.random.default_rng(s)
-    for x in list(r.uniform(-1, 1, 50)) + [0.0, 0.5, -0.5, 1 - 2 ** -53, -(1 - 2 ** -53), 2 ** -60]:
+    special = [0.0, 0.5, -0.5, 1 - 2 ** -53, -(1 - 2 ** -53), 2 ** -60, 127.5 / 128,
+               -127.5 / 128, 0.5 - 2 ** -54, (64 + 0.5 + 127.5 / 256) / 64]
+    for x in list(r.uniform(-1, 1, 50)) + [v for v in special if abs(v) < 1]:
         sig = orc.split_value(float(x), s)
         assert list(sig.astype(int)) == _py_slices(float(x), s)
-        assert np.all(np.abs(sig.astype(int)) <= 127)
-        # truncation to 7s fractional bits: 0 <= |x - sum| < 2^-7s, same sign as x
-        rec = sum(Fraction(int(v)) / 2 ** (7 * (p + 1)) for p, v in enumerate(sig))
-        rem = Fraction(float(x)) - rec
-        assert abs(rem) < Fraction(1, 2 ** (7 * s))
-        assert rem == 0 or (rem > 0) == (x > 0)
+        assert abs(int(sig[0])) <= 65
+        assert np.all(sig[1:].astype(int) >= -128) and np.all(sig[1:].astype(int) <= 127)
+        # digits reproduce x to within half a unit of the last place 2^(2-8s)
+        rec = sum(Fraction(int(v)) * Fraction(2) ** (2 - 8 * (p + 1)) for p, v in enumerate(sig))
+        assert abs(Fraction(float(x)) - rec) <= Fraction(1, 2) * Fraction(2) ** (2 - 8 * s)
+
+
+def test_split_value_carry_cases(orc):
+    # 64 y = 1 + 127.75/256: the first digit is rint(1.499) = 1, the remainder
+    # scaled by 256 is 127.75, whose rint is 128; the carry pass must turn the
+    # digits [1, 128] into [2, -128].
+    y = (1 + 127.75 / 256) / 64
+    assert round((Fraction(y) * 64 - 1) * 256) == 128
+    assert list(orc.split_value(y, 2).astype(int)) == [2, -128]
+    # a chain of carries: 127.5 at every lower level rounds (half to even) to 128
+    x = (0.5 + 127.5 / 256) / 64
+    for s in (2, 3, 5):
+        sig = orc.split_value(x, s).astype(int)
+        assert list(sig) == _py_slices(x, s)
+        assert all(-128 <= d <= 127 for d in sig[1:])
 
 
 def test_split_exponents(orc):
@@ -214,8 +236,8 @@ def test_split_chunk_respects_int32(orc):
             pairs = min(s, t - 1) - max(1, t - s) + 1
             c = orc.split_chunk_tiles(s, t)
             assert c % 2 == 0          # whole 128-byte stages
-            assert pairs * c * 64 * 127 * 127 <= 2 ** 31 - 1
-            assert pairs * (c + 2) * 64 * 127 * 127 > 2 ** 31 - 1
+            assert pairs * c * 64 * 128 * 128 <= 2 ** 31 - 1
+            assert pairs * (c + 2) * 64 * 128 * 128 > 2 ** 31 - 1
 
 
 def _py_split_product(A, B, s):
@@ -237,7 +259,7 @@ def _py_split_product(A, B, s):
                 for p in range(max(1, t - s), min(s, t - 1) + 1):
                     q = t - p
                     g += sum(SA[i][kk][p - 1] * SB[j][kk][q - 1] for kk in range(k))
-                x = fr.rnd(Fraction(g) * Fraction(2) ** (re[i] + ce[j] - 7 * t))
+                x = fr.rnd(Fraction(g) * Fraction(2) ** (re[i] + ce[j] + 4 - 8 * t))
                 v = x if v is None else fr.add(v, x)
             C[i, j] = v
     return C
@@ -254,19 +276,20 @@ def test_split_model_matches_exact_evaluation(orc, s):
 
 def test_split_model_exact_on_7bit_inputs(orc):
     # Entries with at most 7 significant bits relative to their row/column
-    # maximum are represented by the first slice alone: the product is exact.
+    # maximum are represented exactly by the first two digits; with s >= 3 the
+    # pair (2,2) is kept, so the product is exact.
     r = np.random.default_rng(5)
     A = r.integers(-127, 128, size=(31, 90)).astype(np.float64)
     B = r.integers(-127, 128, size=(90, 29)).astype(np.float64) * 2.0 ** -20
     exact = (A.astype(np.int64) @ (B * 2.0 ** 20).astype(np.int64)).astype(np.float64) * 2.0 ** -20
-    for s in (1, 4, 8):
+    for s in (3, 4, 8):
         assert np.array_equal(orc.split_gemm_int8(A, B, s), exact)
 
 
 @pytest.mark.parametrize("s", [2, 4, 6, 8])
 def test_split_model_error_bound(orc, s):
-    # |C - AB| <= 2^(e_i+f_j) [ K (s+4) 2^-7s + (s+2) u K ]: truncation of both
-    # operands, the dropped pairs p+q > s+1, and the FP64 group folds.
+    # |C - AB| <= 2^(e_i+f_j) K [ (s+3) 2^(2-8s) + (s+2) u ]: rounding residuals
+    # of both operands, the dropped pairs p+q > s+1, and the FP64 folds.
     m, n, k = 6, 5, 120
     A = rng_matrix(m, k, 91)
     B = rng_matrix(k, n, 92)
@@ -277,7 +300,7 @@ def test_split_model_error_bound(orc, s):
     for i in range(m):
         for j in range(n):
             err = abs(float(Fraction(C[i, j]) - fr.exact_dot(A[i, :], B[:, j])))
-            bound = 2.0 ** (int(re[i]) + int(ce[j])) * k * ((s + 4) * 2.0 ** (-7 * s) + (s + 2) * u)
+            bound = 2.0 ** (int(re[i]) + int(ce[j])) * k * ((s + 3) * 2.0 ** (2 - 8 * s) + (s + 2) * u)
             assert err <= bound
 
 
@@ -301,3 +324,41 @@ def test_split_model_accumulate_and_nonfinite(orc):
     ok[2] = False
     ok[:, 4] = False
     assert np.all(np.isfinite(out[ok]))
+
+
+# ---- cost-model MLP (double precision) ----------------------------------------------
+
+def test_mlp_gradient_matches_finite_differences(orc):
+    r = np.random.default_rng(0)
+    D, n = 6, 40
+    P = orc.mlp_param_count(D)
+    assert P == D * 64 + 64 + 64 * 64 + 64 + 64 + 1
+    th = r.normal(0, 0.3, P)
+    X = r.normal(0, 1, (n, D))
+    t = r.normal(0, 1, n)
+    loss, g = orc.mlp_gradient(th, X, t)
+    assert loss == pytest.approx(np.mean((orc.mlp_forward(th, X) - t) ** 2), rel=1e-12)
+    idx = r.choice(P, 60, replace=False)
+    idx = np.concatenate([idx, [P - 1, D * 64, D * 64 + 64 + 5, P - 2]])  # b3, b1, W2, W3 entries
+    h = 1e-6
+    for i in idx:
+        tp = th.copy(); tp[i] += h
+        tm = th.copy(); tm[i] -= h
+        fd = (orc.mlp_gradient(tp, X, t)[0] - orc.mlp_gradient(tm, X, t)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), i
+
+
+def test_mlp_forward_closed_form(orc):
+    # With W2 = I-like positive pass-through, the network reduces to a linear
+    # model on positive inputs: y = b3 + sum_j W3_j (b2_j + relu(b1_j + x W1_j)).
+    D = 2
+    P = orc.mlp_param_count(D)
+    th = np.zeros(P)
+    o_b1, o_w2, o_b2, o_w3, o_b3 = D * 64, D * 64 + 64, D * 64 + 64 + 4096, D * 64 + 64 + 4096 + 64, P - 1
+    th[0] = 2.0          # W1[0][0]
+    th[64 + 0] = 3.0     # W1[1][0]
+    th[o_w2 + 0] = 1.0   # W2[0][0]
+    th[o_w3 + 0] = 0.5
+    th[o_b3] = 1.0
+    X = np.array([[1.0, 1.0], [2.0, -1.0], [-5.0, 0.0]])
+    assert np.allclose(orc.mlp_forward(th, X), [1.0 + 0.5 * 5.0, 1.0 + 0.5 * 1.0, 1.0])
