@@ -43,6 +43,9 @@ def parse_args():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target CPU time of the CPU-baseline row slab")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one device copy of this many MiB first (known-size reference "
+                         "for hardware byte counters)")
     return ap.parse_args()
 
 
@@ -103,6 +106,13 @@ def main() -> int:
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
     F = 2.0 * m * n * k
+    if args.copy_probe_mib > 0:
+        src = torch.ones(args.copy_probe_mib * 131072, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
+        del src, dst
 
     def plain_step():
         mgx.ABMultiply(A, B, Cp, m, n, k, k, n, n, 32, stream=stream)

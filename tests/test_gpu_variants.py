@@ -18,8 +18,11 @@ def test_dgemm_tiles(mgx, orc, tile, m, n, k):
     announce(f"dgemm_ex tile={tile} m={m} n={n} k={k}")
     A = orc.fill(m, k, 1, seed=5)
     B = orc.fill(k, n, 1, seed=6)
+    # keep the device tensors referenced for the whole call (a temporary's
+    # memory could be handed to the next allocation)
+    dA, dB = to_dev(A), to_dev(B)
     dC = empty(m, n)
-    mgx._lib.call("mgx_dgemm_ex", tile, 0, to_dev(A).data_ptr(), to_dev(B).data_ptr(),
+    mgx._lib.call("mgx_dgemm_ex", tile, 0, dA.data_ptr(), dB.data_ptr(),
                   dC.data_ptr(), m, n, k, k, n, n, 0)
     C = to_host(dC)
     ref = orc.gemm_kernel_order(A, B)
@@ -28,7 +31,7 @@ def test_dgemm_tiles(mgx, orc, tile, m, n, k):
     # accumulate form
     C0 = orc.fill(m, n, 1, seed=7)
     dC = to_dev(C0)
-    mgx._lib.call("mgx_dgemm_ex", tile, 1, to_dev(A).data_ptr(), to_dev(B).data_ptr(),
+    mgx._lib.call("mgx_dgemm_ex", tile, 1, dA.data_ptr(), dB.data_ptr(),
                   dC.data_ptr(), m, n, k, k, n, n, 0)
     ref2 = orc.gemm_accumulate_order(A, B, C0, k, 0, k)
     absp = orc.gemm_abs_rows(A, B)
@@ -41,9 +44,10 @@ def test_split_tiles_bit_exact(mgx, orc, tile, m, n, k, s):
     announce(f"split_ex tile={tile} m={m} n={n} k={k} s={s}")
     A = orc.fill(m, k, 1, seed=m + k)
     B = orc.fill(k, n, 1, seed=n + k)
+    dA, dB = to_dev(A), to_dev(B)
     dC = empty(m, n)
-    mgx._lib.call("mgx_split_gemm_ex", mgx.FORM_INT8, s, tile, to_dev(A).data_ptr(),
-                  to_dev(B).data_ptr(), dC.data_ptr(), m, n, k, k, n, n, 0, None, 0, 0)
+    mgx._lib.call("mgx_split_gemm_ex", mgx.FORM_INT8, s, tile, dA.data_ptr(),
+                  dB.data_ptr(), dC.data_ptr(), m, n, k, k, n, n, 0, None, 0, 0)
     assert np.array_equal(to_host(dC), orc.split_gemm_int8(A, B, s), equal_nan=True)
 
 
