@@ -159,70 +159,85 @@ __device__ inline void emit_slices16(const double *x, int s, int8_t *dst, int64_
     }
 }
 
-// A slices: Asl[p][row][kp], kp < Kp (zero beyond k).  One lane per 16
-// consecutive k of one row.
+// ---- slice layout ------------------------------------------------------------
+//
+// Slices are stored k-tile blocked: slice p of A is KTC = Kp/64 blocks, block kt
+// holding rows 0..Mp-1 (Mp = M rounded up to 256, padded rows zero) x 64 bytes
+// of k:   Asl[((p * KTC + kt) * Mp + row) * 64 + (k % 64)].
+// B is stored the same way per column of B (n-major, k contiguous).  A 16-row x
+// 64-byte piece of a block is therefore one contiguous KiB in HBM.
+constexpr int ROW_ALIGN = 256;
+
+// One workgroup slices 64 rows x 64 k of A: lane t takes row t/4 and the 16 k
+// values (t%4)*16.. of it, so 64 lanes write one contiguous KiB per slice.
 __global__ __launch_bounds__(256) void slice_rows_kernel(const double *__restrict__ A, int64_t m,
                                                          int64_t k, int64_t lda,
                                                          const int32_t *__restrict__ rexp, int s,
-                                                         int64_t Kp, int8_t *__restrict__ Asl) {
-    const int64_t groups_per_row = Kp / 16;
-    const int64_t total = m * groups_per_row;
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-         g += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = g / groups_per_row;
-        const int64_t k0 = (g - r * groups_per_row) * 16;
-        const int e = rexp[r];
-        const double *row = A + r * lda;
-        double x[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int64_t kk = k0 + i;
-            x[i] = (kk < k && e != EXP_BAD) ? ldexp(row[kk], -e) : 0.0;
-        }
-        emit_slices16(x, s, Asl + r * Kp + k0, m * Kp);
-    }
-}
-
-// B slices, transposed: Bsl[q][col][kp].  A workgroup covers 64 k x 64
-// columns; lane t takes column t/4 and the 16 k values (t%4)*16.. of it, so
-// four lanes write one contiguous 64-byte run of a slice row.
-__global__ __launch_bounds__(256) void slice_cols_kernel(const double *__restrict__ B, int64_t k,
-                                                         int64_t n, int64_t ldb,
-                                                         const int32_t *__restrict__ cexp, int s,
-                                                         int64_t Kp, int8_t *__restrict__ Bsl) {
-    const int64_t c = (int64_t)blockIdx.y * 64 + (threadIdx.x >> 2);
-    const int64_t k0 = (int64_t)blockIdx.x * 64 + (threadIdx.x & 3) * 16;
-    if (c >= n) return;
-    const int e = cexp[c];
+                                                         int64_t Mp, int64_t KTC,
+                                                         int8_t *__restrict__ Asl) {
+    const int64_t kt = blockIdx.x;
+    const int64_t row = (int64_t)blockIdx.y * 64 + (threadIdx.x >> 2);
+    const int kq = threadIdx.x & 3;
+    const int64_t k0 = kt * KT + kq * 16;
+    const int e = row < m ? rexp[row] : EXP_BAD;
     double x[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const int64_t kk = k0 + i;
-        x[i] = (kk < k && e != EXP_BAD) ? ldexp(B[kk * ldb + c], -e) : 0.0;
+        x[i] = (row < m && kk < k && e != EXP_BAD) ? ldexp(A[row * lda + kk], -e) : 0.0;
     }
-    emit_slices16(x, s, Bsl + c * Kp + k0, n * Kp);
+    emit_slices16(x, s, Asl + (kt * Mp + row) * KT + kq * 16, KTC * Mp * KT);
+}
+
+// One workgroup slices 64 columns x 64 k of B: lane t takes column t/4 and the
+// 16 k values (t%4)*16.. of it.
+__global__ __launch_bounds__(256) void slice_cols_kernel(const double *__restrict__ B, int64_t k,
+                                                         int64_t n, int64_t ldb,
+                                                         const int32_t *__restrict__ cexp, int s,
+                                                         int64_t Np, int64_t KTC,
+                                                         int8_t *__restrict__ Bsl) {
+    const int64_t kt = blockIdx.x;
+    const int64_t c = (int64_t)blockIdx.y * 64 + (threadIdx.x >> 2);
+    const int kq = threadIdx.x & 3;
+    const int64_t k0 = kt * KT + kq * 16;
+    const int e = c < n ? cexp[c] : EXP_BAD;
+    double x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int64_t kk = k0 + i;
+        x[i] = (c < n && kk < k && e != EXP_BAD) ? ldexp(B[kk * ldb + c], -e) : 0.0;
+    }
+    emit_slices16(x, s, Bsl + (kt * Np + c) * KT + kq * 16, KTC * Np * KT);
 }
 
 // ---- fused slice-pair GEMM --------------------------------------------------
 //
-// One workgroup of 256 lanes (4 wave64s, 2x2) owns a 256x256 block of C; each
-// wave owns 128x128 as 8x8 INT32 16x16 accumulators (256 registers per lane,
-// held in the accumulation registers).  A and B slice tiles of BK bytes of k
-// are staged through LDS with two buffers; the global loads of step s+1 are
-// issued before the MFMAs of step s.  Rows are padded by 16 bytes so the
-// 16-byte fragment reads of each 8-lane group hit distinct LDS banks.
+// One workgroup of 256 lanes (4 wave64s, 2x2) owns a TB x TB block of C (TB =
+// 256 or 128); each wave owns (TB/2)^2 as 16x16 INT32 MFMA accumulators (256
+// accumulation registers per lane at TB = 256).
 //
-// Steps run in (group t, K chunk, pair p, k tile) order.  At the end of every
+// Operand staging: a ring of NSTAGE LDS slots, each holding the A and B tiles
+// of one 64-byte k stage, is filled by direct global->LDS loads
+// (global_load_lds_dwordx4: lane l of a wave writes its 16 bytes at l*16 of a
+// 1 KiB run, no registers involved).  Stage s+NSTAGE-1 is requested while
+// stage s is multiplied, so NSTAGE-1 stages of loads are in flight.  Stages are
+// published with an explicit vmcnt wait plus a plain s_barrier (a full
+// __syncthreads would also drain the younger stages' loads).
+// In LDS a tile row is 64 bytes; 16-byte chunk c of row r sits at position
+// c ^ ((r >> 2) & 3), so the 16-byte fragment reads of 16 consecutive rows hit
+// 16 distinct bank groups.
+//
+// Steps run in (group t, K chunk, pair p, k stage) order.  At the end of every
 // K chunk the INT32 sums are folded into C in FP64:
-//     x = ldexp((double)G, e_i + f_j - 7t)      (exact)
+//     x = ldexp((double)G, e_i + f_j + 4 - 8t)      (exact)
 //     C = x  or  C_in + x   on the first fold (plain / accumulate form),
 //     C = C + x             on every later fold.
 // so C is read and written once per fold; C stays in HBM between folds.
 
-constexpr int TBK = 128;        // k bytes per LDS stage
-constexpr int TSTR = TBK + 16;  // LDS row stride in bytes
-constexpr int TNT = 256;        // lanes per workgroup
-constexpr int TGROUP_M = 8;     // C block rows per raster group
+constexpr int TNT = 256;          // lanes per workgroup
+constexpr int TGROUP_M = 8;       // C block rows per raster group
+constexpr int NSTAGE = 4;         // LDS ring depth (stages)
+constexpr int CHUNK_GRAIN = 2;    // K chunks are whole multiples of this many 64-k tiles
 
 // Walks the (group t, K chunk, pair p, k tile) steps in execution order.
 struct StepIter {
@@ -240,7 +255,7 @@ struct StepIter {
         pmax = min(s, t - 1);
         const long long np = pmax - pmin + 1;
         chunk = (int)max(1ll, 2147483647ll / (np * SLICE_MAX_PRODUCT * KT));
-        chunk = max(1, chunk / (TBK / KT)) * (TBK / KT);  // whole LDS stages
+        chunk = max(1, chunk / CHUNK_GRAIN) * CHUNK_GRAIN;
         c0 = 0;
         c1 = min(chunk, ktiles);
         p = pmin;
@@ -249,8 +264,7 @@ struct StepIter {
     // Moves past the current step.  Returns true when that step closed an
     // accumulation chunk of the group that was current before the call.
     __device__ bool advance() {
-        kt += TBK / KT;
-        if (kt < c1) return false;
+        if (++kt < c1) return false;
         if (++p <= pmax) {
             kt = c0;
             return false;
@@ -268,18 +282,40 @@ struct StepIter {
     }
 };
 
-// TB = 256 (large tile) or 128 (small tile); waves own (TB/2) x (TB/2).
+// s_waitcnt with vmcnt <= N and the other counters untouched (gfx9 encoding).
+template <int N>
+__device__ inline void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8) | ((N >> 4) << 14));
+}
+
+__device__ inline void block_barrier() {
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// 16-byte direct global->LDS load: lane l of the wave writes its bytes at
+// lds_run + 16*l (lds_run must be wave-uniform).
+__device__ inline void dma16(const int8_t *g, int8_t *lds_run) {
+    __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)lds_run, 16, 0,
+                                     0);
+}
+
 template <int TB>
 __global__ __launch_bounds__(TNT, TB == 256 ? 1 : 2) void ozaki_int8_gemm(
     const int8_t *__restrict__ Asl, const int8_t *__restrict__ Bsl,
     const int32_t *__restrict__ rexp, const int32_t *__restrict__ cexp, double *__restrict__ C,
-    int64_t M, int64_t N, int64_t Kp, int64_t ldc, int s, int64_t nsteps, int accumulate) {
-    constexpr int WT = TB / 2;             // wave tile edge
-    constexpr int FI = WT / 16;            // 16x16 fragments per wave edge
-    constexpr int FOLD_STR = WT + 4;       // ints per row of a wave's fold scratch
-    static_assert(4 * 32 * FOLD_STR * 4 <= 2 * TB * TSTR, "fold scratch must fit one LDS stage");
-    // One contiguous LDS stage per buffer: A tile at offset 0, B tile after it.
-    __shared__ __attribute__((aligned(16))) int8_t lds[2][2 * TB * TSTR];
+    int64_t M, int64_t N, int64_t Mp, int64_t Np, int64_t KTC, int64_t ldc, int s, int64_t nsteps,
+    int accumulate) {
+    constexpr int WT = TB / 2;                 // wave tile edge
+    constexpr int FI = WT / 16;                // 16x16 fragments per wave edge
+    constexpr int TILE = TB * KT;              // bytes of one operand tile per stage
+    constexpr int SLOT = 2 * TILE;             // A tile then B tile
+    constexpr int LPW = TILE / 1024 / 4;       // 1 KiB loads per wave per operand per stage
+    constexpr int FOLD_STR = WT + 4;           // ints per row of a wave's fold scratch
+    constexpr int FOLD_ROWS = 8;               // C rows per fold pass
+    static_assert(4 * FOLD_ROWS * FOLD_STR * 4 <= SLOT, "fold scratch must fit one LDS slot");
+    __shared__ __attribute__((aligned(1024))) int8_t lds[NSTAGE * SLOT];
 
     const int tiles_m = (int)((M + TB - 1) / TB);
     const int tiles_n = (int)((N + TB - 1) / TB);
@@ -301,98 +337,104 @@ __global__ __launch_bounds__(TNT, TB == 256 ? 1 : 2) void ozaki_int8_gemm(
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave >> 1;
     const int wn = wave & 1;
     const int lk = lane >> 4;
     const int lc = lane & 15;
 
-    // Load assignment: 16-byte vector v = tid + 256*i (i < NV) covers row
-    // v / 8, byte offset (v % 8) * 16 of the BK = 128 byte stage.
-    constexpr int VPR = TBK / 16;              // vectors per row
-    constexpr int NV = TB * VPR / TNT;         // vectors per lane per operand
-    const int ld_row0 = tid / VPR;
-    const int ld_off = (tid % VPR) * 16;
-    const int64_t a_stride = M * Kp;
-    const int64_t b_stride = N * Kp;
-
-    v4i ra[NV], rb[NV];
-    const v4i vzero = {0, 0, 0, 0};
-    auto load_step = [&](int p, int q, int kt) {
-        const int8_t *pa = Asl + (int64_t)(p - 1) * a_stride + (int64_t)kt * KT + ld_off;
-        const int8_t *pb = Bsl + (int64_t)(q - 1) * b_stride + (int64_t)kt * KT + ld_off;
-        // Rows past the matrix edge load row 0 and are zeroed by a select, so
-        // the loads need no branches.
+    // Direct-load assignment: load j of this wave fills tile rows
+    // 16*(wave*LPW + j) .. +16; lane l reads row +l/4, 16-byte chunk
+    // (l%4) ^ ((row>>2)&3) = (l%4) ^ ((l>>4)&3), landing at position l%4 of
+    // that row in LDS.  Consecutive loads are 1 KiB apart in HBM and in LDS.
+    const int lane_off = (16 * wave * LPW + (lane >> 2)) * KT + (((lane & 3) ^ ((lane >> 4) & 3)) << 4);
+    const int64_t a_slice = KTC * Mp * KT;
+    const int64_t b_slice = KTC * Np * KT;
+    auto issue = [&](int slot, int p, int q, int kt) {
+        const int8_t *ga = Asl + (int64_t)(p - 1) * a_slice + ((int64_t)kt * Mp + m0) * KT + lane_off;
+        const int8_t *gb = Bsl + (int64_t)(q - 1) * b_slice + ((int64_t)kt * Np + n0) * KT + lane_off;
+        int8_t *la = lds + slot * SLOT + wave * LPW * 1024;
+        int8_t *lb = la + TILE;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            const int64_t ra_row = m0 + ld_row0 + i * (TNT / VPR);
-            const int64_t rb_row = n0 + ld_row0 + i * (TNT / VPR);
-            const v4i va = *(const v4i *)(pa + (ra_row < M ? ra_row : 0) * Kp);
-            const v4i vb = *(const v4i *)(pb + (rb_row < N ? rb_row : 0) * Kp);
-            ra[i] = ra_row < M ? va : vzero;
-            rb[i] = rb_row < N ? vb : vzero;
-        }
-    };
-    auto store_step = [&](int buf) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            const int row = ld_row0 + i * (TNT / VPR);
-            *(v4i *)&lds[buf][row * TSTR + ld_off] = ra[i];
-            *(v4i *)&lds[buf][TB * TSTR + row * TSTR + ld_off] = rb[i];
+        for (int j = 0; j < LPW; ++j) {
+            dma16(ga + j * 1024, la + j * 1024);
+            dma16(gb + j * 1024, lb + j * 1024);
         }
     };
 
     v4i acc[FI][FI];
-#pragma unroll
-    for (int i = 0; i < FI; ++i)
-#pragma unroll
-        for (int j = 0; j < FI; ++j) acc[i][j] = vzero;
+    const v4i vzero = {0, 0, 0, 0};
 
-    const int ktiles = (int)(Kp / KT);
-    StepIter it;
-    it.start(s, ktiles);
-    load_step(it.p, it.t - it.p, it.kt);
-    store_step(0);
-    __syncthreads();
+    const int ktiles = (int)KTC;
+    StepIter comp, load;
+    comp.start(s, ktiles);
+    load.start(s, ktiles);
+    // prologue: stages 0 .. NSTAGE-2
+    for (int st = 0; st < NSTAGE - 1 && st < nsteps; ++st) {
+        issue(st, load.p, load.t - load.p, load.kt);
+        load.advance();
+    }
     bool first_fold = true;
+    constexpr int LPS = 2 * LPW;   // load instructions per wave per stage
 
-    for (int64_t step = 0; step < nsteps; ++step) {
-        const int buf = (int)(step & 1);
-        const int t_cur = it.t;
-        const bool fold = it.advance();
-        const bool more = step + 1 < nsteps;
-        if (more) load_step(it.p, it.t - it.p, it.kt);
+    int64_t step = 0;
+    while (step < nsteps) {
+        // one accumulation chunk: steps until the iterator closes the chunk
+#pragma unroll
+        for (int i = 0; i < FI; ++i)
+#pragma unroll
+            for (int j = 0; j < FI; ++j) acc[i][j] = vzero;
+        int t_cur;
+        bool fold;
+        do {
+            const int slot = (int)(step % NSTAGE);
+            // this wave's loads of `step` are done once at most the younger
+            // stages' loads are outstanding
+            const int64_t younger = min((int64_t)(NSTAGE - 2), nsteps - 1 - step);
+            if (younger >= 2)
+                wait_vmcnt<2 * LPS>();
+            else if (younger == 1)
+                wait_vmcnt<LPS>();
+            else
+                wait_vmcnt<0>();
+            block_barrier();   // every wave's loads of `step` are in LDS; slot step-1 is free
+            if (step + NSTAGE - 1 < nsteps) {
+                issue((int)((step + NSTAGE - 1) % NSTAGE), load.p, load.t - load.p, load.kt);
+                load.advance();
+            }
+            t_cur = comp.t;
+            fold = comp.advance();
 
-#pragma unroll 1
-        for (int kk = 0; kk < TBK / KT; ++kk) {
+            const int8_t *la = lds + slot * SLOT;
+            const int8_t *lb = la + TILE;
             v4i af[FI], bf[FI];
 #pragma unroll
-            for (int i = 0; i < FI; ++i)
-                af[i] = *(const v4i *)&lds[buf][(wm * WT + i * 16 + lc) * TSTR + kk * KT + 16 * lk];
+            for (int i = 0; i < FI; ++i) {
+                const int r = wm * WT + i * 16 + lc;
+                af[i] = *(const v4i *)(la + r * KT + ((lk ^ ((r >> 2) & 3)) << 4));
+            }
 #pragma unroll
-            for (int j = 0; j < FI; ++j)
-                bf[j] = *(const v4i *)&lds[buf][TB * TSTR + (wn * WT + j * 16 + lc) * TSTR + kk * KT +
-                                                16 * lk];
+            for (int j = 0; j < FI; ++j) {
+                const int r = wn * WT + j * 16 + lc;
+                bf[j] = *(const v4i *)(lb + r * KT + ((lk ^ ((r >> 2) & 3)) << 4));
+            }
 #pragma unroll
             for (int i = 0; i < FI; ++i)
 #pragma unroll
                 for (int j = 0; j < FI; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
-        }
+            ++step;
+        } while (!fold);
 
-        // The prefetched slices go to the idle LDS buffer before the fold, so
-        // they do not stay live across it.
-        if (more) store_step(buf ^ 1);
-
-        if (fold) {
-            // Fold through LDS: the buffer read by this step is free once every
-            // wave has passed the barrier.  FI/2 passes; in pass q each wave
-            // dumps its INT32 accumulators of fragment rows 2q, 2q+1 (32 rows
-            // x WT columns, rows padded to FOLD_STR ints) and then folds them
-            // into C row by row, WT/64 adjacent columns per lane.
-            constexpr int CPL = WT / 64;   // columns per lane in the fold
+        // Fold through LDS slot (step-1) % NSTAGE: its operands are consumed
+        // once every wave passes the barrier; the in-flight loads target other
+        // slots.  Pass (i, h) dumps fragment row i, rows 8h..8h+7 (lanes with
+        // lk/2 == h), then folds them into C, WT/64 adjacent columns per lane.
+        {
+            constexpr int CPL = WT / 64;
             __syncthreads();
-            int *fold_buf = (int *)&lds[buf][0] + wave * (32 * FOLD_STR);
+            const int slot = (int)((step - 1) % NSTAGE);
+            int *fold_buf = (int *)(lds + slot * SLOT) + wave * (FOLD_ROWS * FOLD_STR);
             int64_t fcol[CPL];
             int ecol[CPL];
 #pragma unroll
@@ -401,43 +443,40 @@ __global__ __launch_bounds__(TNT, TB == 256 ? 1 : 2) void ozaki_int8_gemm(
                 ecol[c] = fcol[c] < N ? cexp[fcol[c]] : 0;
             }
 #pragma unroll
-            for (int q = 0; q < FI / 2; ++q) {
-                // INT32 layout: lane l, register r holds row 4*(l/16)+r, column l%16.
+            for (int i = 0; i < FI; ++i) {
 #pragma unroll
-                for (int ii = 0; ii < 2; ++ii)
+                for (int h = 0; h < 2; ++h) {
+                    // INT32 layout: lane l, register r holds row 4*(l/16)+r, column l%16.
+                    if ((lk >> 1) == h) {
 #pragma unroll
-                    for (int j = 0; j < FI; ++j)
+                        for (int j = 0; j < FI; ++j)
 #pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            fold_buf[(ii * 16 + 4 * lk + r) * FOLD_STR + j * 16 + lc] =
-                                acc[2 * q + ii][j][r];
-                __syncthreads();
+                            for (int r = 0; r < 4; ++r)
+                                fold_buf[((lk & 1) * 4 + r) * FOLD_STR + j * 16 + lc] = acc[i][j][r];
+                    }
+                    __syncthreads();
 #pragma unroll 2
-                for (int rho = 0; rho < 32; ++rho) {
-                    const int64_t row = m0 + wm * WT + 32 * q + rho;
-                    if (row >= M) break;
-                    const int er = rexp[row];
-                    double *c_row = C + row * ldc;
+                    for (int rho = 0; rho < FOLD_ROWS; ++rho) {
+                        const int64_t row = m0 + wm * WT + i * 16 + h * 8 + rho;
+                        if (row >= M) break;
+                        const int er = rexp[row];
+                        double *c_row = C + row * ldc;
 #pragma unroll
-                    for (int c = 0; c < CPL; ++c) {
-                        if (fcol[c] < N) {
-                            const int g = fold_buf[rho * FOLD_STR + CPL * lane + c];
-                            const double x = (er == EXP_BAD || ecol[c] == EXP_BAD)
-                                                 ? __longlong_as_double(0x7ff8000000000000ll)
-                                                 : ldexp((double)g, er + ecol[c] + 4 - 8 * t_cur);
-                            c_row[fcol[c]] = (first_fold && !accumulate) ? x : c_row[fcol[c]] + x;
+                        for (int c = 0; c < CPL; ++c) {
+                            if (fcol[c] < N) {
+                                const int g = fold_buf[rho * FOLD_STR + CPL * lane + c];
+                                const double x = (er == EXP_BAD || ecol[c] == EXP_BAD)
+                                                     ? __longlong_as_double(0x7ff8000000000000ll)
+                                                     : ldexp((double)g, er + ecol[c] + 4 - 8 * t_cur);
+                                c_row[fcol[c]] = (first_fold && !accumulate) ? x : c_row[fcol[c]] + x;
+                            }
                         }
                     }
+                    __syncthreads();
                 }
-                __syncthreads();
             }
-#pragma unroll
-            for (int i = 0; i < FI; ++i)
-#pragma unroll
-                for (int j = 0; j < FI; ++j) acc[i][j] = vzero;
             first_fold = false;
         }
-        __syncthreads();
     }
 }
 
@@ -458,7 +497,7 @@ unsigned grid1d(int64_t total, int threads, int64_t cap = 65536) {
 }
 
 struct SplitLayout {
-    int64_t Kp;
+    int64_t Kp, KTC, Mp, Np;
     size_t a_off, b_off, rmax_off, cmax_off, rexp_off, cexp_off, total;
 };
 
@@ -471,8 +510,11 @@ SplitLayout split_layout(int s, int64_t m, int64_t n, int64_t k) {
         off += (bytes + 255) / 256 * 256;
         return o;
     };
-    L.a_off = take((size_t)s * m * L.Kp);
-    L.b_off = take((size_t)s * n * L.Kp);
+    L.KTC = L.Kp / KT;
+    L.Mp = round_up(m < 1 ? 1 : m, ROW_ALIGN);
+    L.Np = round_up(n < 1 ? 1 : n, ROW_ALIGN);
+    L.a_off = take((size_t)s * L.Mp * L.Kp);
+    L.b_off = take((size_t)s * L.Np * L.Kp);
     L.rmax_off = take((size_t)(m + n) * 8);
     L.cmax_off = L.rmax_off + (size_t)m * 8;
     L.rexp_off = take((size_t)m * 4);
@@ -575,24 +617,30 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     if ((rc = mgx::launch_status("exponents_kernel")) != MGX_OK) return rc;
     if ((rc = mark(1)) != MGX_OK) return rc;
 
-    hipLaunchKernelGGL(slice_rows_kernel, dim3(grid1d(m * (L.Kp / 16), 256, 1 << 20)), dim3(256),
-                       0, st, A, m, k, lda, rexp, slices, L.Kp, Asl);
-    if ((rc = mgx::launch_status("slice_rows_kernel")) != MGX_OK) return rc;
     {
-        const int64_t gy = (n + 63) / 64;
+        const int64_t gy = L.Mp / 64;
+        MGX_ARG_CHECK(gy <= 65535 && L.KTC <= 0x7fffffffll, "m or k too large for the row slicer");
+        hipLaunchKernelGGL(slice_rows_kernel, dim3((unsigned)L.KTC, (unsigned)gy), dim3(256), 0, st,
+                           A, m, k, lda, rexp, slices, L.Mp, L.KTC, Asl);
+        if ((rc = mgx::launch_status("slice_rows_kernel")) != MGX_OK) return rc;
+    }
+    {
+        const int64_t gy = L.Np / 64;
         MGX_ARG_CHECK(gy <= 65535, "n too large for the column slicer");
-        hipLaunchKernelGGL(slice_cols_kernel, dim3((unsigned)(L.Kp / 64), (unsigned)gy), dim3(256),
-                           0, st, B, k, n, ldb, cexp, slices, L.Kp, Bsl);
+        hipLaunchKernelGGL(slice_cols_kernel, dim3((unsigned)L.KTC, (unsigned)gy), dim3(256), 0, st,
+                           B, k, n, ldb, cexp, slices, L.Np, L.KTC, Bsl);
         if ((rc = mgx::launch_status("slice_cols_kernel")) != MGX_OK) return rc;
     }
     if ((rc = mark(2)) != MGX_OK) return rc;
-    const int64_t nsteps = (L.Kp / TBK) * (int64_t)(slices * (slices + 1) / 2);
+    const int64_t nsteps = L.KTC * (int64_t)(slices * (slices + 1) / 2);
     if (tile == MGX_TILE_LARGE)
         hipLaunchKernelGGL(ozaki_int8_gemm<256>, dim3((unsigned)tiles), dim3(TNT), 0, st, Asl, Bsl,
-                           rexp, cexp, C, m, n, L.Kp, ldc, slices, nsteps, accumulate ? 1 : 0);
+                           rexp, cexp, C, m, n, L.Mp, L.Np, L.KTC, ldc, slices, nsteps,
+                           accumulate ? 1 : 0);
     else
         hipLaunchKernelGGL(ozaki_int8_gemm<128>, dim3((unsigned)tiles), dim3(TNT), 0, st, Asl, Bsl,
-                           rexp, cexp, C, m, n, L.Kp, ldc, slices, nsteps, accumulate ? 1 : 0);
+                           rexp, cexp, C, m, n, L.Mp, L.Np, L.KTC, ldc, slices, nsteps,
+                           accumulate ? 1 : 0);
     if ((rc = mgx::launch_status("ozaki_int8_gemm")) != MGX_OK) return rc;
     return mark(3);
 }
