@@ -6,10 +6,12 @@ error (there is no fallback implementation of any entry point).
 from __future__ import annotations
 
 import ctypes
+import os
 import re
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libmgx.so"
+# MGX_LIBRARY names an alternative build of the same library.
+LIB_PATH = Path(os.environ.get("MGX_LIBRARY") or Path(__file__).resolve().parent / "libmgx.so")
 HEADER_PATH = Path(__file__).resolve().parent.parent / "include" / "mgx.h"
 
 MGX_OK = 0

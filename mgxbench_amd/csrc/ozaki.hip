@@ -212,9 +212,11 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double *__restric
 
 // ---- fused slice-pair GEMM --------------------------------------------------
 //
-// One workgroup of 256 lanes (4 wave64s, 2x2) owns a TB x TB block of C (TB =
-// 256 or 128); each wave owns (TB/2)^2 as 16x16 INT32 MFMA accumulators (256
-// accumulation registers per lane at TB = 256).
+// One workgroup owns a TB x TB block of C.  TB = 256: 8 wave64s (2 x 4), two
+// per SIMD, each holding a 128 x 64 INT32 accumulator tile (128 accumulation
+// registers per lane); TB = 128: 4 waves (2 x 2) of 64 x 64, two workgroups per
+// CU.  Two waves per SIMD let one wave's LDS reads, barrier wait and load
+// issue overlap the other's MFMAs.
 //
 // Operand staging: a ring of NSTAGE LDS slots, each holding the A and B tiles
 // of one 64-byte k stage, is filled by direct global->LDS loads
@@ -224,8 +226,7 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double *__restric
 // published with an explicit vmcnt wait plus a plain s_barrier (a full
 // __syncthreads would also drain the younger stages' loads).
 // In LDS a tile row is 64 bytes; 16-byte chunk c of row r sits at position
-// c ^ ((r >> 2) & 3), so the 16-byte fragment reads of 16 consecutive rows hit
-// 16 distinct bank groups.
+// c ^ swz(r) (below), which keeps the fragment reads free of bank conflicts.
 //
 // Steps run in (group t, K chunk, pair p, k stage) order.  At the end of every
 // K chunk the INT32 sums are folded into C in FP64:
@@ -234,51 +235,84 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double *__restric
 //     C = C + x             on every later fold.
 // so C is read and written once per fold; C stays in HBM between folds.
 
-constexpr int TNT = 256;          // lanes per workgroup
 constexpr int TGROUP_M = 8;       // C block rows per raster group
 constexpr int NSTAGE = 4;         // LDS ring depth (stages)
 constexpr int CHUNK_GRAIN = 2;    // K chunks are whole multiples of this many 64-k tiles
 
-// Walks the (group t, K chunk, pair p, k tile) steps in execution order.
-struct StepIter {
-    int s, ktiles;
-    int t, pmin, pmax, p, c0, c1, kt, chunk;
+// Group-t accumulation chunk in 64-deep k tiles: the largest whole multiple
+// of CHUNK_GRAIN tiles whose INT32 sums over all np pairs of the group stay
+// below 2^31.
+__device__ inline int chunk_tiles(int np) {
+    int chunk = (int)max(1ll, 2147483647ll / (np * SLICE_MAX_PRODUCT * KT));
+    return max(1, chunk / CHUNK_GRAIN) * CHUNK_GRAIN;
+}
 
-    __device__ void start(int s_, int ktiles_) {
-        s = s_;
-        ktiles = ktiles_;
-        t = 2;
-        begin_group();
+// Position of the next stage to load, walked in the product's step order
+// (group t, K chunk, pair p, k tile).  The common advance adds one k tile to
+// both operand pointers; group, chunk and pair changes take the slow path.
+// Past the last step the cursor stays on the last stage, so the load that
+// keeps the per-step load count uniform re-reads a valid tile into a slot no
+// step reads again.
+struct LoadCursor {
+    const int8_t *ga, *gb;   // tile base of the current stage (no lane offset)
+    int kt, c1;              // current k tile, end of the current chunk
+    int t, p, pmax, c0, chunk;
+    int s, ktiles;
+    int64_t mkt, nkt;        // bytes per k tile of one A / B slice tile row band
+    int64_t a_slice, b_slice;
+    const int8_t *abase, *bbase;
+
+    __device__ void place() {
+        ga = abase + (int64_t)(p - 1) * a_slice + kt * mkt;
+        gb = bbase + (int64_t)(t - p - 1) * b_slice + kt * nkt;
     }
     __device__ void begin_group() {
-        pmin = max(1, t - s);
+        p = max(1, t - s);
         pmax = min(s, t - 1);
-        const long long np = pmax - pmin + 1;
-        chunk = (int)max(1ll, 2147483647ll / (np * SLICE_MAX_PRODUCT * KT));
-        chunk = max(1, chunk / CHUNK_GRAIN) * CHUNK_GRAIN;
+        chunk = chunk_tiles(pmax - p + 1);
         c0 = 0;
         c1 = min(chunk, ktiles);
-        p = pmin;
         kt = 0;
     }
-    // Moves past the current step.  Returns true when that step closed an
-    // accumulation chunk of the group that was current before the call.
-    __device__ bool advance() {
-        if (++kt < c1) return false;
+    __device__ void start(int s_, int ktiles_, const int8_t *abase_, const int8_t *bbase_,
+                          int64_t mkt_, int64_t nkt_, int64_t a_slice_, int64_t b_slice_) {
+        s = s_;
+        ktiles = ktiles_;
+        abase = abase_;
+        bbase = bbase_;
+        mkt = mkt_;
+        nkt = nkt_;
+        a_slice = a_slice_;
+        b_slice = b_slice_;
+        t = 2;
+        begin_group();
+        place();
+    }
+    __device__ void advance() {
+        if (++kt < c1) {
+            ga += mkt;
+            gb += nkt;
+            return;
+        }
+        slow_advance();
+    }
+    __device__ void slow_advance() {
         if (++p <= pmax) {
             kt = c0;
-            return false;
-        }
-        c0 = c1;
-        if (c0 < ktiles) {
+        } else if (c1 < ktiles) {
+            c0 = c1;
             c1 = min(c0 + chunk, ktiles);
-            p = pmin;
+            p = max(1, t - s);
             kt = c0;
-            return true;
+        } else if (t <= s) {
+            ++t;
+            begin_group();
+        } else {
+            kt = c1 - 1;   // past the end: stay on the last stage
+            p = pmax;
+            return;
         }
-        ++t;
-        if (t <= s + 1) begin_group();
-        return true;
+        place();
     }
 };
 
@@ -288,6 +322,12 @@ __device__ inline void wait_vmcnt() {
     static_assert(N >= 0 && N < 64, "vmcnt range");
     __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8) | ((N >> 4) << 14));
 }
+
+// LDS position of the 16-byte chunks of a 64-byte tile row r: chunk c sits at
+// c ^ swz(r).  A ds_read_b128 is served 8 lanes (128 bytes) at a time; the 8
+// rows of such a group then cover all 32 banks.  Row blocks are 16-row
+// aligned, so swz(row) depends on row % 16 only.
+__device__ inline int swz(int r) { return (r >> 1) & 3; }
 
 __device__ inline void block_barrier() {
     __builtin_amdgcn_s_barrier();
@@ -301,21 +341,41 @@ __device__ inline void dma16(const int8_t *g, int8_t *lds_run) {
                                      0);
 }
 
-template <int TB>
-__global__ __launch_bounds__(TNT, TB == 256 ? 1 : 2) void ozaki_int8_gemm(
+// TB x TB block tile computed by WGM x WGN waves (wave tile TB/WGM x TB/WGN).
+// The 256 tile runs 8 waves, two per SIMD, so one wave's LDS reads, barrier
+// and load issue overlap the other wave's MFMAs.
+template <int TB, int WGM, int WGN>
+struct TileShape {
+    static constexpr int NW = WGM * WGN;           // waves per workgroup
+    static constexpr int THREADS = 64 * NW;
+    static constexpr int WTM = TB / WGM;           // wave tile rows
+    static constexpr int WTN = TB / WGN;           // wave tile columns
+    static constexpr int FI = WTM / 16;            // 16x16 fragments along rows
+    static constexpr int FJ = WTN / 16;            // ... along columns
+    static constexpr int TILE = TB * KT;           // bytes of one operand tile per stage
+    static constexpr int SLOT = 2 * TILE;          // A tile then B tile
+    static constexpr int LPW = TILE / 1024 / NW;   // 1 KiB loads per wave per operand per stage
+    static constexpr int FOLD_ROWS = 8;            // C rows per fold pass
+    static constexpr int FOLD_STR = WTN + 4;       // ints per row of a wave's fold scratch
+    static constexpr int FOLD_WAVE = FOLD_ROWS * FOLD_STR;
+    static constexpr int LDS_BYTES = NSTAGE * SLOT + NW * FOLD_WAVE * 4;
+    static_assert(WTN == 64, "the fold maps one C column per lane");
+    static_assert(LPW >= 1 && LPW * NW * 1024 == TILE, "loads must cover the tile");
+    static_assert(LDS_BYTES <= 160 * 1024, "LDS budget");
+};
+using ShapeLarge = TileShape<256, 2, 4>;
+using ShapeSmall = TileShape<128, 2, 2>;
+
+template <int TB, int WGM, int WGN, int MINB>
+__global__ __launch_bounds__(64 * WGM * WGN, MINB) void ozaki_int8_gemm(
     const int8_t *__restrict__ Asl, const int8_t *__restrict__ Bsl,
     const int32_t *__restrict__ rexp, const int32_t *__restrict__ cexp, double *__restrict__ C,
-    int64_t M, int64_t N, int64_t Mp, int64_t Np, int64_t KTC, int64_t ldc, int s, int64_t nsteps,
-    int accumulate) {
-    constexpr int WT = TB / 2;                 // wave tile edge
-    constexpr int FI = WT / 16;                // 16x16 fragments per wave edge
-    constexpr int TILE = TB * KT;              // bytes of one operand tile per stage
-    constexpr int SLOT = 2 * TILE;             // A tile then B tile
-    constexpr int LPW = TILE / 1024 / 4;       // 1 KiB loads per wave per operand per stage
-    constexpr int FOLD_STR = WT + 4;           // ints per row of a wave's fold scratch
-    constexpr int FOLD_ROWS = 8;               // C rows per fold pass
-    static_assert(4 * FOLD_ROWS * FOLD_STR * 4 <= SLOT, "fold scratch must fit one LDS slot");
-    __shared__ __attribute__((aligned(1024))) int8_t lds[NSTAGE * SLOT];
+    int64_t M, int64_t N, int64_t Mp, int64_t Np, int64_t KTC, int64_t ldc, int s, int accumulate) {
+    using S = TileShape<TB, WGM, WGN>;
+    constexpr int FI = S::FI, FJ = S::FJ, TILE = S::TILE, SLOT = S::SLOT, LPW = S::LPW;
+    constexpr int FOLD_ROWS = S::FOLD_ROWS, FOLD_STR = S::FOLD_STR;
+    // operand ring, then a private fold scratch region per wave
+    __shared__ __attribute__((aligned(1024))) int8_t lds[S::LDS_BYTES];
 
     const int tiles_m = (int)((M + TB - 1) / TB);
     const int tiles_n = (int)((N + TB - 1) / TB);
@@ -328,120 +388,102 @@ __global__ __launch_bounds__(TNT, TB == 256 ? 1 : 2) void ozaki_int8_gemm(
         const int g = b / per_group;
         const int first_m = g * TGROUP_M;
         const int gsize = min(tiles_m - first_m, TGROUP_M);
-        const int r = b - g * per_group;
-        tm = first_m + r % gsize;
-        tn = r / gsize;
+        tm = first_m + (b % per_group) % gsize;
+        tn = (b % per_group) / gsize;
     }
     const int64_t m0 = (int64_t)tm * TB;
     const int64_t n0 = (int64_t)tn * TB;
-
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wm = wave >> 1;
-    const int wn = wave & 1;
+    const int wm = wave / WGN;
+    const int wn = wave % WGN;
     const int lk = lane >> 4;
     const int lc = lane & 15;
 
     // Direct-load assignment: load j of this wave fills tile rows
     // 16*(wave*LPW + j) .. +16; lane l reads row +l/4, 16-byte chunk
-    // (l%4) ^ ((row>>2)&3) = (l%4) ^ ((l>>4)&3), landing at position l%4 of
-    // that row in LDS.  Consecutive loads are 1 KiB apart in HBM and in LDS.
-    const int lane_off = (16 * wave * LPW + (lane >> 2)) * KT + (((lane & 3) ^ ((lane >> 4) & 3)) << 4);
-    const int64_t a_slice = KTC * Mp * KT;
-    const int64_t b_slice = KTC * Np * KT;
-    auto issue = [&](int slot, int p, int q, int kt) {
-        const int8_t *ga = Asl + (int64_t)(p - 1) * a_slice + ((int64_t)kt * Mp + m0) * KT + lane_off;
-        const int8_t *gb = Bsl + (int64_t)(q - 1) * b_slice + ((int64_t)kt * Np + n0) * KT + lane_off;
-        int8_t *la = lds + slot * SLOT + wave * LPW * 1024;
-        int8_t *lb = la + TILE;
-#pragma unroll
-        for (int j = 0; j < LPW; ++j) {
-            dma16(ga + j * 1024, la + j * 1024);
-            dma16(gb + j * 1024, lb + j * 1024);
-        }
-    };
+    // (l%4) ^ swz(l/4), landing at position l%4 of that row in LDS.
+    // Consecutive loads are 1 KiB apart in HBM and in LDS.
+    const int lane_off = (16 * wave * LPW + (lane >> 2)) * KT + (((lane & 3) ^ swz(lane >> 2)) << 4);
 
-    v4i acc[FI][FI];
+    v4i acc[FI][FJ];
     const v4i vzero = {0, 0, 0, 0};
-
-    const int ktiles = (int)KTC;
-    StepIter comp, load;
-    comp.start(s, ktiles);
-    load.start(s, ktiles);
-    // prologue: stages 0 .. NSTAGE-2
-    for (int st = 0; st < NSTAGE - 1 && st < nsteps; ++st) {
-        issue(st, load.p, load.t - load.p, load.kt);
-        load.advance();
-    }
-    bool first_fold = true;
     constexpr int LPS = 2 * LPW;   // load instructions per wave per stage
 
-    int64_t step = 0;
-    while (step < nsteps) {
-        // one accumulation chunk: steps until the iterator closes the chunk
+    const int ktiles = (int)KTC;
+    LoadCursor cur;
+    cur.start(s, ktiles, Asl + m0 * KT, Bsl + n0 * KT, Mp * KT, Np * KT, KTC * Mp * KT,
+              KTC * Np * KT);
+    const uint32_t voff = (uint32_t)lane_off;
+    int8_t *const lds_w = lds + wave * LPW * 1024;   // this wave's load runs
+    auto issue = [&](int slot) {
+        int8_t *la = lds_w + slot * SLOT;
 #pragma unroll
-        for (int i = 0; i < FI; ++i)
-#pragma unroll
-            for (int j = 0; j < FI; ++j) acc[i][j] = vzero;
-        int t_cur;
-        bool fold;
-        do {
-            const int slot = (int)(step % NSTAGE);
-            // this wave's loads of `step` are done once at most the younger
-            // stages' loads are outstanding
-            const int64_t younger = min((int64_t)(NSTAGE - 2), nsteps - 1 - step);
-            if (younger >= 2)
-                wait_vmcnt<2 * LPS>();
-            else if (younger == 1)
-                wait_vmcnt<LPS>();
-            else
-                wait_vmcnt<0>();
-            block_barrier();   // every wave's loads of `step` are in LDS; slot step-1 is free
-            if (step + NSTAGE - 1 < nsteps) {
-                issue((int)((step + NSTAGE - 1) % NSTAGE), load.p, load.t - load.p, load.kt);
-                load.advance();
-            }
-            t_cur = comp.t;
-            fold = comp.advance();
+        for (int j = 0; j < LPW; ++j) {
+            dma16(cur.ga + voff + j * 1024, la + j * 1024);
+            dma16(cur.gb + voff + j * 1024, la + TILE + j * 1024);
+        }
+        cur.advance();
+    };
+    // prologue: stages 0 .. NSTAGE-2.  Every step then issues exactly one
+    // stage, so NSTAGE-1 stages are in flight at each step's wait.
+    for (int st = 0; st < NSTAGE - 1; ++st) issue(st);
+    bool first_fold = true;
+    int *fold_buf = (int *)(lds + NSTAGE * SLOT) + wave * S::FOLD_WAVE;
+    int slot = 0;
 
-            const int8_t *la = lds + slot * SLOT;
-            const int8_t *lb = la + TILE;
-            v4i af[FI], bf[FI];
-#pragma unroll
-            for (int i = 0; i < FI; ++i) {
-                const int r = wm * WT + i * 16 + lc;
-                af[i] = *(const v4i *)(la + r * KT + ((lk ^ ((r >> 2) & 3)) << 4));
-            }
-#pragma unroll
-            for (int j = 0; j < FI; ++j) {
-                const int r = wn * WT + j * 16 + lc;
-                bf[j] = *(const v4i *)(lb + r * KT + ((lk ^ ((r >> 2) & 3)) << 4));
-            }
+    for (int t = 2; t <= s + 1; ++t) {
+        const int np = min(s, t - 1) - max(1, t - s) + 1;
+        const int chunk = chunk_tiles(np);
+        for (int c0 = 0; c0 < ktiles; c0 += chunk) {
+            const int nchunk = min(chunk, ktiles - c0) * np;   // steps in this chunk
 #pragma unroll
             for (int i = 0; i < FI; ++i)
 #pragma unroll
-                for (int j = 0; j < FI; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
-            ++step;
-        } while (!fold);
+                for (int j = 0; j < FJ; ++j) acc[i][j] = vzero;
+            for (int n = 0; n < nchunk; ++n) {
+                // this wave's loads of the current stage are done once at most
+                // the NSTAGE-2 younger stages are outstanding; its reads of the
+                // previous stage completed before the previous MFMAs
+                wait_vmcnt<(NSTAGE - 2) * LPS>();
+                block_barrier();   // every wave's loads are in LDS; the previous slot is free
+                issue((slot + NSTAGE - 1) & (NSTAGE - 1));
 
-        // Fold through LDS slot (step-1) % NSTAGE: its operands are consumed
-        // once every wave passes the barrier; the in-flight loads target other
-        // slots.  Pass (i, h) dumps fragment row i, rows 8h..8h+7 (lanes with
-        // lk/2 == h), then folds them into C, WT/64 adjacent columns per lane.
-        {
-            constexpr int CPL = WT / 64;
-            __syncthreads();
-            const int slot = (int)((step - 1) % NSTAGE);
-            int *fold_buf = (int *)(lds + slot * SLOT) + wave * (FOLD_ROWS * FOLD_STR);
-            int64_t fcol[CPL];
-            int ecol[CPL];
+                const int8_t *la = lds + slot * SLOT;
+                const int8_t *lb = la + TILE;
+                v4i af[FI], bf[FJ];
 #pragma unroll
-            for (int c = 0; c < CPL; ++c) {
-                fcol[c] = n0 + wn * WT + CPL * lane + c;
-                ecol[c] = fcol[c] < N ? cexp[fcol[c]] : 0;
+                for (int i = 0; i < FI; ++i) {
+                    const int r = wm * S::WTM + i * 16 + lc;
+                    af[i] = *(const v4i *)(la + r * KT + ((lk ^ swz(r)) << 4));
+                }
+#pragma unroll
+                for (int j = 0; j < FJ; ++j) {
+                    const int r = wn * S::WTN + j * 16 + lc;
+                    bf[j] = *(const v4i *)(lb + r * KT + ((lk ^ swz(r)) << 4));
+                }
+                // all fragment reads are issued before the first MFMA (the
+                // scheduler would otherwise reuse one register per A fragment
+                // and wait for each read in turn)
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int i = 0; i < FI; ++i)
+#pragma unroll
+                    for (int j = 0; j < FJ; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+                slot = (slot + 1) & (NSTAGE - 1);
             }
+            const int t_cur = t;
+
+        // Fold: each wave dumps its INT32 accumulators to its private LDS
+        // scratch, FOLD_ROWS C rows per pass (pass (i, h): fragment row i, rows
+        // 8h..8h+7, held by the lanes with lk/2 == h), and folds them into C,
+        // one column per lane.  A wave's LDS operations complete in order, so
+        // no workgroup barrier is needed.
+        {
+            const int64_t fcol = n0 + wn * S::WTN + lane;
+            const int ecol = fcol < N ? cexp[fcol] : 0;
 #pragma unroll
             for (int i = 0; i < FI; ++i) {
 #pragma unroll
@@ -449,33 +491,33 @@ __global__ __launch_bounds__(TNT, TB == 256 ? 1 : 2) void ozaki_int8_gemm(
                     // INT32 layout: lane l, register r holds row 4*(l/16)+r, column l%16.
                     if ((lk >> 1) == h) {
 #pragma unroll
-                        for (int j = 0; j < FI; ++j)
+                        for (int j = 0; j < FJ; ++j)
 #pragma unroll
                             for (int r = 0; r < 4; ++r)
                                 fold_buf[((lk & 1) * 4 + r) * FOLD_STR + j * 16 + lc] = acc[i][j][r];
                     }
-                    __syncthreads();
+                    asm volatile("" ::: "memory");
 #pragma unroll 2
                     for (int rho = 0; rho < FOLD_ROWS; ++rho) {
-                        const int64_t row = m0 + wm * WT + i * 16 + h * 8 + rho;
+                        const int64_t row = m0 + wm * S::WTM + i * 16 + h * 8 + rho;
                         if (row >= M) break;
                         const int er = rexp[row];
                         double *c_row = C + row * ldc;
-#pragma unroll
-                        for (int c = 0; c < CPL; ++c) {
-                            if (fcol[c] < N) {
-                                const int g = fold_buf[rho * FOLD_STR + CPL * lane + c];
-                                const double x = (er == EXP_BAD || ecol[c] == EXP_BAD)
-                                                     ? __longlong_as_double(0x7ff8000000000000ll)
-                                                     : ldexp((double)g, er + ecol[c] + 4 - 8 * t_cur);
-                                c_row[fcol[c]] = (first_fold && !accumulate) ? x : c_row[fcol[c]] + x;
-                            }
+                        if (fcol < N) {
+                            const int g = fold_buf[rho * FOLD_STR + lane];
+                            const double x = (er == EXP_BAD || ecol == EXP_BAD)
+                                                 ? __longlong_as_double(0x7ff8000000000000ll)
+                                                 : ldexp((double)g, er + ecol + 4 - 8 * t_cur);
+                            c_row[fcol] = (first_fold && !accumulate) ? x : c_row[fcol] + x;
                         }
                     }
-                    __syncthreads();
+                    asm volatile("" ::: "memory");
                 }
             }
             first_fold = false;
+            // the fold's C traffic retires before the stage counting resumes
+            wait_vmcnt<0>();
+        }
         }
     }
 }
@@ -632,15 +674,14 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
         if ((rc = mgx::launch_status("slice_cols_kernel")) != MGX_OK) return rc;
     }
     if ((rc = mark(2)) != MGX_OK) return rc;
-    const int64_t nsteps = L.KTC * (int64_t)(slices * (slices + 1) / 2);
     if (tile == MGX_TILE_LARGE)
-        hipLaunchKernelGGL(ozaki_int8_gemm<256>, dim3((unsigned)tiles), dim3(TNT), 0, st, Asl, Bsl,
-                           rexp, cexp, C, m, n, L.Mp, L.Np, L.KTC, ldc, slices, nsteps,
-                           accumulate ? 1 : 0);
+        hipLaunchKernelGGL((ozaki_int8_gemm<256, 2, 4, 1>), dim3((unsigned)tiles),
+                           dim3(ShapeLarge::THREADS), 0, st, Asl, Bsl, rexp, cexp, C, m, n, L.Mp,
+                           L.Np, L.KTC, ldc, slices, accumulate ? 1 : 0);
     else
-        hipLaunchKernelGGL(ozaki_int8_gemm<128>, dim3((unsigned)tiles), dim3(TNT), 0, st, Asl, Bsl,
-                           rexp, cexp, C, m, n, L.Mp, L.Np, L.KTC, ldc, slices, nsteps,
-                           accumulate ? 1 : 0);
+        hipLaunchKernelGGL((ozaki_int8_gemm<128, 2, 2, 2>), dim3((unsigned)tiles),
+                           dim3(ShapeSmall::THREADS), 0, st, Asl, Bsl, rexp, cexp, C, m, n, L.Mp,
+                           L.Np, L.KTC, ldc, slices, accumulate ? 1 : 0);
     if ((rc = mgx::launch_status("ozaki_int8_gemm")) != MGX_OK) return rc;
     return mark(3);
 }
