@@ -37,8 +37,13 @@ enum {
 /* Split-precision forms. */
 enum {
     MGX_FORM_FP64 = 0,         /* plain FP64 MFMA (v_mfma_f64_16x16x4_f64) */
-    MGX_FORM_INT8 = 1,         /* Ozaki split: INT8 slices, exact INT32 MFMA accumulation */
-    MGX_FORM_BF16 = 2          /* Ozaki split: BF16 slices, exact FP32 MFMA accumulation */
+    MGX_FORM_INT8 = 1,         /* Ozaki scheme I: INT8 digit slices, exact INT32 MFMA sums of
+                                  the s(s+1)/2 slice pairs, FP64 folds */
+    MGX_FORM_BF16 = 2,         /* reserved (BF16 slices); not built: every call returns
+                                  MGX_ERR_UNSUPPORTED */
+    MGX_FORM_INT8_MOD = 3      /* Ozaki scheme II: INT8 residues of the 2^beta-scaled integer
+                                  operands modulo `slices` coprime moduli, one exact INT32 MFMA
+                                  product per modulus, Chinese remaindering to FP64 */
 };
 
 /* Block-tile variants of the product kernels (autotuner knob). */
@@ -115,7 +120,12 @@ int mgx_validate_constant(const double *dC, int64_t rows, int64_t cols,
  * >= mgx_split_workspace_bytes(...); pass workspace == NULL to use a
  * library-owned per-device buffer grown on demand.
  * Inputs must be finite; a row of A or column of B holding Inf/NaN
- * produces NaN in the corresponding row/column of C. */
+ * produces NaN in the corresponding row/column of C.
+ * MGX_FORM_INT8_MOD: `slices` is the number of moduli (2..16); the result
+ * is ldexp(RN(A'B'), e_i + f_j - 2 beta) with A', B' the 2^beta-scaled
+ * integer operands (see mgx_split_mod_beta), so each output is rounded once.
+ * Its events: [2] before the residue products, [3] after them (the
+ * reconstruction into C follows [3]). */
 size_t mgx_split_workspace_bytes(int form, int slices,
                                  int64_t m, int64_t n, int64_t k);
 int mgx_split_gemm(int form, int slices,
@@ -146,8 +156,15 @@ int mgx_split_gemm_ex(int form, int slices, int tile,
                       int accumulate, void *workspace, size_t workspace_bytes,
                       void *stream);
 
-/* Supported slice-count range for a split form (inclusive). */
+/* Supported slice-count range for a split form (inclusive); for
+ * MGX_FORM_INT8_MOD the count is the number of moduli. */
 int mgx_split_slices_range(int form, int *min_slices, int *max_slices);
+
+/* Scaling exponent beta of MGX_FORM_INT8_MOD for `moduli` moduli and depth k:
+ * A and B are quantised to beta-bit integers per row / column (the form's
+ * only error); the largest beta <= 53 with 2 k 4^beta < m_0 ... m_{n-1}.
+ * Returns -1 when the moduli cannot hold the product (the call would fail). */
+int mgx_split_mod_beta(int moduli, int64_t k);
 
 /* ---- accuracy report ----------------------------------------------------
  * Error of a computed device product C (m x n) against a double-double

@@ -17,12 +17,12 @@ MgxError instead of aborting the process.
 from __future__ import annotations
 
 from . import _lib
-from ._lib import FORM_BF16, FORM_FP64, FORM_INT8, MgxError, lib
+from ._lib import FORM_BF16, FORM_FP64, FORM_INT8, FORM_INT8_MOD, MgxError, lib
 
 __all__ = [
     "ABMultiply", "ABMultiplyAccumulateAsync", "validate_matrix_C", "split_gemm",
     "split_gemm_events", "split_workspace_bytes", "error_report", "summa_run", "fill", "fill_exponents", "scale_rows_cols", "version", "peak",
-    "FORM_FP64", "FORM_INT8", "FORM_BF16", "MgxError", "lib",
+    "FORM_FP64", "FORM_INT8", "FORM_INT8_MOD", "FORM_BF16", "MgxError", "lib",
 ]
 
 

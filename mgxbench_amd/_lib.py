@@ -24,6 +24,7 @@ MGX_ERR_ALLOC = 5
 FORM_FP64 = 0
 FORM_INT8 = 1
 FORM_BF16 = 2
+FORM_INT8_MOD = 3
 
 _i64 = ctypes.c_int64
 _vp = ctypes.c_void_p
@@ -80,6 +81,7 @@ _SIGNATURES = {
                                      _i64, _int, _vp, ctypes.c_size_t, _vp,
                                      ctypes.POINTER(_vp)]),
     "mgx_split_slices_range": (_int, [_int, ctypes.POINTER(_int), ctypes.POINTER(_int)]),
+    "mgx_split_mod_beta": (_int, [_int, _i64]),
     "mgx_error_report": (_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _i64,
                                 ctypes.POINTER(ErrorStats)]),
     "mgx_summa_run": (_int, [ctypes.POINTER(SummaParams), _vp, _i64, _vp, _i64, _vp, _i64,

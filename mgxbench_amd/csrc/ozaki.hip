@@ -1,42 +1,60 @@
-// FP64-accurate split-precision GEMM (Ozaki scheme I) on the INT8 matrix
-// cores of gfx950.
+// FP64-accurate split-precision GEMM on the INT8 matrix cores of gfx950, in
+// two forms that share one INT8 product kernel.
 //
-// No counterpart in the reference: it computes the same local product as
+// No counterpart in the reference: both compute the same local product as
 // ABMultiply / ABMultiplyAccumulateAsync (1D-row-block/mulmat_kernel.cu:86,
 // 148) through a different arithmetic.
 //
-// Algorithm (INT8 form):
-//  1. e_i = frexp exponent of max_k |A_ik| (so |A_ik| < 2^e_i), f_j likewise
-//     for column j of B.  A row/column holding Inf/NaN gets a sentinel and
-//     its outputs become NaN.
-//  2. x = A_ik * 2^-e_i in (-1,1) is cut into s signed digits by round to
+// Common first step: e_i = frexp exponent of max_k |A_ik| (so |A_ik| < 2^e_i),
+// f_j likewise for column j of B.  A row/column holding Inf/NaN gets a
+// sentinel and its outputs become NaN.  Operands are stored k-tile blocked,
+// int8, B per column (n-major, k contiguous).
+//
+// Digit-slice form (MGX_FORM_INT8, Ozaki scheme I):
+//  1. x = A_ik 2^-e_i in (-1,1) is cut into s signed digits by round to
 //     nearest: sigma_1 = rint(x 2^6), then sigma_p = rint(256 r) of the
 //     previous remainder r, with a carry pass that keeps every digit in
 //     [-128, 127] (slice_digits).  Place values are 2^(2-8p); the residual
-//     after s slices is at most 2^(1-8s).  Every step is exact in FP64.  B is
-//     sliced per column with f_j and stored transposed (n-major, k contiguous).
-//  3. For each weight group t = 2..s+1 (slice pairs p+q = t, place value
+//     after s slices is at most 2^(1-8s).  Every step is exact in FP64.
+//  2. For each weight group t = 2..s+1 (slice pairs p+q = t, place value
 //     2^(4-8t)), the INT8 MFMA accumulates G_t = sum_{p+q=t} sum_k sigmaA_p
-//     sigmaB_q exactly in INT32.  The K range is cut into chunks of `chunk`
-//     64-deep tiles (whole LDS stages), chosen so that
+//     sigmaB_q exactly in INT32 over K chunks with
 //     pairs(t) * chunk * 64 * 128^2 < 2^31.
-//  4. After each chunk the sums are folded into C in FP64:
+//  3. After each chunk the sums are folded into C in FP64:
 //     x = ldexp((double)G, e_i + f_j + 4 - 8t) (exact); the first fold writes
 //     C = x (or C_in + x for the accumulate form), later folds C = C + x.
-// Pairs with p+q > s+1 are dropped; with the residuals they bound the error
-// by about (s+2) 2^(2-8s) K 2^(e_i+f_j).
+//
+// Modular form (MGX_FORM_INT8_MOD, Ozaki scheme II; modular.hpp):
+//  1. A'_ik = rint(A_ik 2^(beta-e_i)) (|A'| <= 2^beta), B' likewise, with
+//     beta the largest exponent <= 53 such that 2 k 4^beta < P, the product
+//     of the first n moduli.  Operands are the residues of A', B' modulo
+//     each m_l, as int8.
+//  2. For each modulus the INT8 MFMA forms the residue product exactly in
+//     INT32 (reduced mod m_l between K chunks of at most 2046 tiles) and
+//     stores it mod m_l as one byte per output.
+//  3. A reconstruction kernel recovers the integer C' = A'B' from its n
+//     residues (Garner), rounds it once to FP64 and writes
+//     C = ldexp(C', e_i + f_j - 2 beta) (or C_in + that).
+// The modular form needs n products instead of s(s+1)/2 and its only error
+// is the quantisation of A and B to beta bits.
 #include "mgx_internal.hpp"
+#include "modular.hpp"
 
 #include <algorithm>
 
 namespace {
 
+using mgx::modular::MAX_MODULI;
+using mgx::modular::MIN_MODULI;
+using mgx::modular::MODULI;
+
 constexpr int KT = 64;             // k depth of one INT8 MFMA step / LDS stage
-constexpr int K_ALIGN = 256;       // slice rows are zero-padded to this multiple
+constexpr int K_ALIGN = 256;       // operand rows are zero-padded to this multiple
 constexpr int EXP_BAD = 0x40000000;
 constexpr int MIN_SLICES = 1;
 constexpr int MAX_SLICES = 9;
 constexpr long long SLICE_MAX_PRODUCT = 128ll * 128ll;  // largest |sigma_p sigma_q|
+constexpr int MOD_CHUNK_TILES = 2046;   // 2046 * 64 * 128^2 + 256 < 2^31
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
@@ -159,22 +177,49 @@ __device__ inline void emit_slices16(const double *x, int s, int8_t *dst, int64_
     }
 }
 
-// ---- slice layout ------------------------------------------------------------
+// Writes the n residues (modular form) of 16 consecutive integer-valued
+// scaled values x[0..15] as 16 bytes each.  dst points at modulus 0;
+// consecutive moduli are `plane_stride` apart.
+__device__ inline void emit_residues16(const double *x, int n, int8_t *dst, int64_t plane_stride) {
+#pragma unroll
+    for (int l = 0; l < MAX_MODULI; ++l) {
+        if (l < n) {
+            unsigned int words[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                words[i / 4] |= ((unsigned int)(mgx::modular::residue_int8(x[i], l) & 0xff))
+                                << (8 * (i % 4));
+            const v4i packed = {(int)words[0], (int)words[1], (int)words[2], (int)words[3]};
+            *(v4i *)(dst + (int64_t)l * plane_stride) = packed;
+        }
+    }
+}
+
+// ---- operand layout ------------------------------------------------------------
 //
-// Slices are stored k-tile blocked: slice p of A is KTC = Kp/64 blocks, block kt
-// holding rows 0..Mp-1 (Mp = M rounded up to 256, padded rows zero) x 64 bytes
-// of k:   Asl[((p * KTC + kt) * Mp + row) * 64 + (k % 64)].
-// B is stored the same way per column of B (n-major, k contiguous).  A 16-row x
-// 64-byte piece of a block is therefore one contiguous KiB in HBM.
+// Operands are stored k-tile blocked: plane p (slice p or modulus p) of A is
+// KTC = Kp/64 blocks, block kt holding rows 0..Mp-1 (Mp = M rounded up to 256,
+// padded rows zero) x 64 bytes of k:
+//     Aop[((p * KTC + kt) * Mp + row) * 64 + (k % 64)].
+// B is stored the same way per column of B (n-major, k contiguous).  A 16-row
+// x 64-byte piece of a block is therefore one contiguous KiB in HBM.
 constexpr int ROW_ALIGN = 256;
 
-// One workgroup slices 64 rows x 64 k of A: lane t takes row t/4 and the 16 k
-// values (t%4)*16.. of it, so 64 lanes write one contiguous KiB per slice.
-__global__ __launch_bounds__(256) void slice_rows_kernel(const double *__restrict__ A, int64_t m,
-                                                         int64_t k, int64_t lda,
-                                                         const int32_t *__restrict__ rexp, int s,
-                                                         int64_t Mp, int64_t KTC,
-                                                         int8_t *__restrict__ Asl) {
+// Per-element operand value: scheme I scales into (-1,1), scheme II to an
+// integer of at most beta bits.
+template <bool MOD>
+__device__ inline double scaled_operand(double a, int e, int beta) {
+    return MOD ? rint(ldexp(a, beta - e)) : ldexp(a, -e);
+}
+
+// One workgroup converts 64 rows x 64 k of A: lane t takes row t/4 and the 16
+// k values (t%4)*16.. of it, so 64 lanes write one contiguous KiB per plane.
+template <bool MOD>
+__global__ __launch_bounds__(256) void operand_rows_kernel(const double *__restrict__ A,
+                                                           int64_t m, int64_t k, int64_t lda,
+                                                           const int32_t *__restrict__ rexp,
+                                                           int planes, int beta, int64_t Mp,
+                                                           int64_t KTC, int8_t *__restrict__ Aop) {
     const int64_t kt = blockIdx.x;
     const int64_t row = (int64_t)blockIdx.y * 64 + (threadIdx.x >> 2);
     const int kq = threadIdx.x & 3;
@@ -184,18 +229,24 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double *__restric
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const int64_t kk = k0 + i;
-        x[i] = (row < m && kk < k && e != EXP_BAD) ? ldexp(A[row * lda + kk], -e) : 0.0;
+        x[i] = (row < m && kk < k && e != EXP_BAD) ? scaled_operand<MOD>(A[row * lda + kk], e, beta)
+                                                   : 0.0;
     }
-    emit_slices16(x, s, Asl + (kt * Mp + row) * KT + kq * 16, KTC * Mp * KT);
+    int8_t *dst = Aop + (kt * Mp + row) * KT + kq * 16;
+    if (MOD)
+        emit_residues16(x, planes, dst, KTC * Mp * KT);
+    else
+        emit_slices16(x, planes, dst, KTC * Mp * KT);
 }
 
-// One workgroup slices 64 columns x 64 k of B: lane t takes column t/4 and the
-// 16 k values (t%4)*16.. of it.
-__global__ __launch_bounds__(256) void slice_cols_kernel(const double *__restrict__ B, int64_t k,
-                                                         int64_t n, int64_t ldb,
-                                                         const int32_t *__restrict__ cexp, int s,
-                                                         int64_t Np, int64_t KTC,
-                                                         int8_t *__restrict__ Bsl) {
+// One workgroup converts 64 columns x 64 k of B: lane t takes column t/4 and
+// the 16 k values (t%4)*16.. of it.
+template <bool MOD>
+__global__ __launch_bounds__(256) void operand_cols_kernel(const double *__restrict__ B,
+                                                           int64_t k, int64_t n, int64_t ldb,
+                                                           const int32_t *__restrict__ cexp,
+                                                           int planes, int beta, int64_t Np,
+                                                           int64_t KTC, int8_t *__restrict__ Bop) {
     const int64_t kt = blockIdx.x;
     const int64_t c = (int64_t)blockIdx.y * 64 + (threadIdx.x >> 2);
     const int kq = threadIdx.x & 3;
@@ -205,86 +256,95 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double *__restric
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const int64_t kk = k0 + i;
-        x[i] = (c < n && kk < k && e != EXP_BAD) ? ldexp(B[kk * ldb + c], -e) : 0.0;
+        x[i] = (c < n && kk < k && e != EXP_BAD) ? scaled_operand<MOD>(B[kk * ldb + c], e, beta)
+                                                 : 0.0;
     }
-    emit_slices16(x, s, Bsl + (kt * Np + c) * KT + kq * 16, KTC * Np * KT);
+    int8_t *dst = Bop + (kt * Np + c) * KT + kq * 16;
+    if (MOD)
+        emit_residues16(x, planes, dst, KTC * Np * KT);
+    else
+        emit_slices16(x, planes, dst, KTC * Np * KT);
 }
 
-// ---- fused slice-pair GEMM --------------------------------------------------
+// ---- INT8 product kernel ------------------------------------------------------
 //
 // One workgroup owns a TB x TB block of C.  TB = 256: 8 wave64s (2 x 4), two
-// per SIMD, each holding a 128 x 64 INT32 accumulator tile (128 accumulation
-// registers per lane); TB = 128: 4 waves (2 x 2) of 64 x 64, two workgroups per
-// CU.  Two waves per SIMD let one wave's LDS reads, barrier wait and load
-// issue overlap the other's MFMAs.
+// per SIMD, each holding a 128 x 64 INT32 accumulator tile; TB = 128: 4 waves
+// (2 x 2) of 64 x 64, two workgroups per CU.  Two waves per SIMD let one
+// wave's LDS reads, barrier wait and load issue overlap the other's MFMAs.
 //
 // Operand staging: a ring of NSTAGE LDS slots, each holding the A and B tiles
 // of one 64-byte k stage, is filled by direct global->LDS loads
 // (global_load_lds_dwordx4: lane l of a wave writes its 16 bytes at l*16 of a
-// 1 KiB run, no registers involved).  Stage s+NSTAGE-1 is requested while
-// stage s is multiplied, so NSTAGE-1 stages of loads are in flight.  Stages are
-// published with an explicit vmcnt wait plus a plain s_barrier (a full
-// __syncthreads would also drain the younger stages' loads).
-// In LDS a tile row is 64 bytes; 16-byte chunk c of row r sits at position
-// c ^ swz(r) (below), which keeps the fragment reads free of bank conflicts.
+// 1 KiB run, no registers involved).  Every step requests exactly one stage
+// NSTAGE-1 ahead (NSTAGE-1 stages in flight).  Stages are published with an
+// explicit vmcnt wait plus a plain s_barrier (a full __syncthreads would also
+// drain the younger stages' loads).  In LDS a tile row is 64 bytes; 16-byte
+// chunk c of row r sits at position c ^ swz(r), which keeps the fragment
+// reads free of bank conflicts.
 //
-// Steps run in (group t, K chunk, pair p, k stage) order.  At the end of every
-// K chunk the INT32 sums are folded into C in FP64:
-//     x = ldexp((double)G, e_i + f_j + 4 - 8t)      (exact)
-//     C = x  or  C_in + x   on the first fold (plain / accumulate form),
-//     C = C + x             on every later fold.
-// so C is read and written once per fold; C stays in HBM between folds.
+// Steps run group by group (scheme I: weight group t, slice pairs p + q = t;
+// scheme II: one group per modulus), K chunk by K chunk, pair by pair, k tile
+// by k tile.  A chunk is a whole even number of k tiles.
 
 constexpr int TGROUP_M = 8;       // C block rows per raster group
-constexpr int NSTAGE = 4;         // LDS ring depth (stages)
 constexpr int CHUNK_GRAIN = 2;    // K chunks are whole multiples of this many 64-k tiles
+constexpr int NSTAGE = 5;         // LDS ring depth (stages)
 
-// Group-t accumulation chunk in 64-deep k tiles: the largest whole multiple
-// of CHUNK_GRAIN tiles whose INT32 sums over all np pairs of the group stay
-// below 2^31.
+// Scheme I group-t accumulation chunk in 64-deep k tiles: the largest whole
+// multiple of CHUNK_GRAIN tiles whose INT32 sums over all np pairs of the
+// group stay below 2^31.
 __device__ inline int chunk_tiles(int np) {
-    int chunk = (int)max(1ll, 2147483647ll / (np * SLICE_MAX_PRODUCT * KT));
+    // floor(floor(a / b) / c) == floor(a / (b c)) for positive integers
+    constexpr int PER_PAIR = (int)(2147483647ll / (SLICE_MAX_PRODUCT * KT));
+    const int chunk = max(1, PER_PAIR / np);
     return max(1, chunk / CHUNK_GRAIN) * CHUNK_GRAIN;
 }
 
-// Position of the next stage to load, walked in the product's step order
-// (group t, K chunk, pair p, k tile).  The common advance adds one k tile to
-// both operand pointers; group, chunk and pair changes take the slow path.
-// Past the last step the cursor stays on the last stage, so the load that
-// keeps the per-step load count uniform re-reads a valid tile into a slot no
-// step reads again.
+// Position of the next stage to load, walked in the product's step order.
+// Scheme I: groups t = 2..s+1, pairs p = max(1,t-s)..min(s,t-1), A plane p-1,
+// B plane t-p-1.  Scheme II: groups t = 0..s-1, one pair, A and B plane t.
+// The common advance adds one k tile to both operand pointers; group, chunk
+// and pair changes take the slow path.  Past the last step the cursor stays
+// on the last stage, so the load that keeps the per-step load count uniform
+// re-reads a valid tile into a slot no step reads again.
+template <bool MOD>
 struct LoadCursor {
     const int8_t *ga, *gb;   // tile base of the current stage (no lane offset)
     int kt, c1;              // current k tile, end of the current chunk
     int t, p, pmax, c0, chunk;
     int s, ktiles;
-    int64_t mkt, nkt;        // bytes per k tile of one A / B slice tile row band
-    int64_t a_slice, b_slice;
+    int64_t mkt, nkt;        // bytes per k tile of one A / B operand plane
+    int64_t a_plane, b_plane;
     const int8_t *abase, *bbase;
 
     __device__ void place() {
-        ga = abase + (int64_t)(p - 1) * a_slice + kt * mkt;
-        gb = bbase + (int64_t)(t - p - 1) * b_slice + kt * nkt;
+        const int pa = MOD ? t : p - 1;
+        const int pb = MOD ? t : t - p - 1;
+        ga = abase + (int64_t)pa * a_plane + kt * mkt;
+        gb = bbase + (int64_t)pb * b_plane + kt * nkt;
     }
+    __device__ int first_pair() const { return MOD ? t : max(1, t - s); }
     __device__ void begin_group() {
-        p = max(1, t - s);
-        pmax = min(s, t - 1);
-        chunk = chunk_tiles(pmax - p + 1);
+        p = first_pair();
+        pmax = MOD ? t : min(s, t - 1);
+        chunk = MOD ? MOD_CHUNK_TILES : chunk_tiles(pmax - p + 1);
         c0 = 0;
         c1 = min(chunk, ktiles);
         kt = 0;
     }
+    __device__ bool last_group() const { return MOD ? t >= s - 1 : t > s; }
     __device__ void start(int s_, int ktiles_, const int8_t *abase_, const int8_t *bbase_,
-                          int64_t mkt_, int64_t nkt_, int64_t a_slice_, int64_t b_slice_) {
+                          int64_t mkt_, int64_t nkt_, int64_t a_plane_, int64_t b_plane_) {
         s = s_;
         ktiles = ktiles_;
         abase = abase_;
         bbase = bbase_;
         mkt = mkt_;
         nkt = nkt_;
-        a_slice = a_slice_;
-        b_slice = b_slice_;
-        t = 2;
+        a_plane = a_plane_;
+        b_plane = b_plane_;
+        t = MOD ? 0 : 2;
         begin_group();
         place();
     }
@@ -302,9 +362,9 @@ struct LoadCursor {
         } else if (c1 < ktiles) {
             c0 = c1;
             c1 = min(c0 + chunk, ktiles);
-            p = max(1, t - s);
+            p = first_pair();
             kt = c0;
-        } else if (t <= s) {
+        } else if (!last_group()) {
             ++t;
             begin_group();
         } else {
@@ -341,9 +401,86 @@ __device__ inline void dma16(const int8_t *g, int8_t *lds_run) {
                                      0);
 }
 
-// TB x TB block tile computed by WGM x WGN waves (wave tile TB/WGM x TB/WGN).
-// The 256 tile runs 8 waves, two per SIMD, so one wave's LDS reads, barrier
-// and load issue overlap the other wave's MFMAs.
+
+// Folds one wave's INT32 accumulators into C (scheme I): x = ldexp(G, e_i +
+// f_j + shift) (NaN when a row or column exponent is EXP_BAD), C = x on an
+// overwriting fold, else C = C + x.  In the INT32 MFMA layout lane l,
+// register r of fragment (i, j) holds row 16i + 4(l/16) + r, column
+// 16j + l%16 of the wave tile, so one load or store instruction covers four
+// 128-byte row segments of C.  row0 / col0 are this lane's first row and
+// column.  EDGE adds the bounds checks that tiles crossing the matrix edge
+// need.
+template <int FI, int FJ, bool EDGE>
+__device__ __forceinline__ void fold_tile(const v4i (&acc)[FI][FJ], double *C, int64_t ldc,
+                                          const int32_t *rexp, const int32_t *cexp, int64_t M,
+                                          int64_t N, int64_t row0, int64_t col0, int shift,
+                                          bool overwrite) {
+    int ecol[FJ];
+#pragma unroll
+    for (int j = 0; j < FJ; ++j) ecol[j] = (!EDGE || col0 + 16 * j < N) ? cexp[col0 + 16 * j] : 0;
+    double *cptr = C + row0 * ldc + col0;   // row row0 + 16i while fragment row i is folded
+    int64_t row = row0;
+#pragma unroll
+    for (int i = 0; i < FI; ++i) {
+        int erow[4];
+        double cv[FJ][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const bool rok = !EDGE || row + r < M;
+            erow[r] = rok ? rexp[row + r] : 0;
+#pragma unroll
+            for (int j = 0; j < FJ; ++j) {
+                const bool ok = rok && (!EDGE || col0 + 16 * j < N);
+                cv[j][r] = (!overwrite && ok) ? cptr[r * ldc + 16 * j] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+            for (int j = 0; j < FJ; ++j) {
+                const bool ok = !EDGE || (row + r < M && col0 + 16 * j < N);
+                if (ok) {
+                    const double x = (erow[r] == EXP_BAD || ecol[j] == EXP_BAD)
+                                         ? __longlong_as_double(0x7ff8000000000000ll)
+                                         : ldexp((double)acc[i][j][r], erow[r] + ecol[j] + shift);
+                    cptr[r * ldc + 16 * j] = overwrite ? x : cv[j][r] + x;
+                }
+            }
+        }
+        cptr += 16 * ldc;
+        row += 16;
+    }
+}
+
+// acc mod m in [0, m) for every accumulator (scheme II).  q = floor(a / m) is
+// taken from the FP64 quotient, which is within 2^-21 of the exact one, so
+// one correction step each way suffices.
+template <int FI, int FJ>
+__device__ __forceinline__ void reduce_mod(v4i (&acc)[FI][FJ], int m) {
+    const double inv = 1.0 / (double)m;
+#pragma unroll
+    for (int i = 0; i < FI; ++i)
+#pragma unroll
+        for (int j = 0; j < FJ; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int a = acc[i][j][r];
+                int v = a - (int)floor((double)a * inv) * m;
+                if (v < 0) v += m;
+                if (v >= m) v -= m;
+                acc[i][j][r] = v;
+            }
+}
+
+// Layout of the scheme II product residues: per modulus, per C block tile,
+// per wave, QPL = FI FJ / 4 runs of 1 KiB; run q holds, for every lane, the
+// 16 bytes of fragments 4q..4q+3 (fragment f = i FJ + j, byte 4 (f % 4) + r
+// within the lane's 16 bytes is register r).
+__device__ __forceinline__ int64_t residue_run(int l, int64_t tiles, int64_t tile, int nw,
+                                               int wave, int qpl, int q) {
+    return ((((int64_t)l * tiles + tile) * nw + wave) * qpl + q) * 1024;
+}
+
 template <int TB, int WGM, int WGN>
 struct TileShape {
     static constexpr int NW = WGM * WGN;           // waves per workgroup
@@ -352,35 +489,45 @@ struct TileShape {
     static constexpr int WTN = TB / WGN;           // wave tile columns
     static constexpr int FI = WTM / 16;            // 16x16 fragments along rows
     static constexpr int FJ = WTN / 16;            // ... along columns
+    static constexpr int QPL = FI * FJ / 4;        // 16-byte residue runs per lane
     static constexpr int TILE = TB * KT;           // bytes of one operand tile per stage
     static constexpr int SLOT = 2 * TILE;          // A tile then B tile
     static constexpr int LPW = TILE / 1024 / NW;   // 1 KiB loads per wave per operand per stage
-    static constexpr int FOLD_ROWS = 8;            // C rows per fold pass
-    static constexpr int FOLD_STR = WTN + 4;       // ints per row of a wave's fold scratch
-    static constexpr int FOLD_WAVE = FOLD_ROWS * FOLD_STR;
-    static constexpr int LDS_BYTES = NSTAGE * SLOT + NW * FOLD_WAVE * 4;
-    static_assert(WTN == 64, "the fold maps one C column per lane");
+    static constexpr int LDS_BYTES = NSTAGE * SLOT;
     static_assert(LPW >= 1 && LPW * NW * 1024 == TILE, "loads must cover the tile");
     static_assert(LDS_BYTES <= 160 * 1024, "LDS budget");
+    static_assert(FI % 2 == 0 && (FI * FJ) % 4 == 0, "fragment grouping");
 };
 using ShapeLarge = TileShape<256, 2, 4>;
 using ShapeSmall = TileShape<128, 2, 2>;
 
-template <int TB, int WGM, int WGN, int MINB>
-__global__ __launch_bounds__(64 * WGM * WGN, MINB) void ozaki_int8_gemm(
-    const int8_t *__restrict__ Asl, const int8_t *__restrict__ Bsl,
-    const int32_t *__restrict__ rexp, const int32_t *__restrict__ cexp, double *__restrict__ C,
-    int64_t M, int64_t N, int64_t Mp, int64_t Np, int64_t KTC, int64_t ldc, int s, int accumulate) {
+struct ProductArgs {
+    const int8_t *A, *B;          // operand planes, k-tile blocked
+    const int32_t *rexp, *cexp;   // row / column exponents (scheme I fold)
+    double *C;                    // scheme I output
+    int64_t ldc;
+    uint8_t *R;                   // scheme II product residues
+    int64_t M, N, Mp, Np, KTC;
+    int s;                        // slices (scheme I) or moduli (scheme II)
+    int accumulate;
+};
+
+template <int TB, int WGM, int WGN, bool MOD, int MINB>
+__global__ __launch_bounds__(64 * WGM * WGN, MINB) void int8_product_kernel(ProductArgs a) {
     using S = TileShape<TB, WGM, WGN>;
     constexpr int FI = S::FI, FJ = S::FJ, TILE = S::TILE, SLOT = S::SLOT, LPW = S::LPW;
-    constexpr int FOLD_ROWS = S::FOLD_ROWS, FOLD_STR = S::FOLD_STR;
-    // operand ring, then a private fold scratch region per wave
+    // operand ring
     __shared__ __attribute__((aligned(1024))) int8_t lds[S::LDS_BYTES];
 
+    const int64_t M = a.M, N = a.N;
+    const int s = a.s;
     const int tiles_m = (int)((M + TB - 1) / TB);
     const int tiles_n = (int)((N + TB - 1) / TB);
     int tm, tn;
     {
+        // XCD-aware remap: consecutive workgroups go to different XCDs, so
+        // tile ranges are dealt out per XCD (when the count divides by 8),
+        // then rastered in groups of TGROUP_M block rows for L2 reuse.
         const int nb = tiles_m * tiles_n;
         int b = blockIdx.x;
         if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
@@ -393,6 +540,7 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINB) void ozaki_int8_gemm(
     }
     const int64_t m0 = (int64_t)tm * TB;
     const int64_t n0 = (int64_t)tn * TB;
+    const bool full_tile = m0 + TB <= M && n0 + TB <= N;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -411,10 +559,10 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINB) void ozaki_int8_gemm(
     const v4i vzero = {0, 0, 0, 0};
     constexpr int LPS = 2 * LPW;   // load instructions per wave per stage
 
-    const int ktiles = (int)KTC;
-    LoadCursor cur;
-    cur.start(s, ktiles, Asl + m0 * KT, Bsl + n0 * KT, Mp * KT, Np * KT, KTC * Mp * KT,
-              KTC * Np * KT);
+    const int ktiles = (int)a.KTC;
+    LoadCursor<MOD> cur;
+    cur.start(s, ktiles, a.A + m0 * KT, a.B + n0 * KT, a.Mp * KT, a.Np * KT, a.KTC * a.Mp * KT,
+              a.KTC * a.Np * KT);
     const uint32_t voff = (uint32_t)lane_off;
     int8_t *const lds_w = lds + wave * LPW * 1024;   // this wave's load runs
     auto issue = [&](int slot) {
@@ -429,95 +577,196 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINB) void ozaki_int8_gemm(
     // prologue: stages 0 .. NSTAGE-2.  Every step then issues exactly one
     // stage, so NSTAGE-1 stages are in flight at each step's wait.
     for (int st = 0; st < NSTAGE - 1; ++st) issue(st);
-    bool first_fold = true;
-    int *fold_buf = (int *)(lds + NSTAGE * SLOT) + wave * S::FOLD_WAVE;
-    int slot = 0;
+    int pslot = NSTAGE - 1;   // ring slot of the most recently published stage
 
-    for (int t = 2; t <= s + 1; ++t) {
-        const int np = min(s, t - 1) - max(1, t - s) + 1;
-        const int chunk = chunk_tiles(np);
-        for (int c0 = 0; c0 < ktiles; c0 += chunk) {
-            const int nchunk = min(chunk, ktiles - c0) * np;   // steps in this chunk
+    // Publishes the next stage: this wave's loads of it are done once at most
+    // the NSTAGE-2 younger stages are outstanding; after the barrier every
+    // wave's loads are in LDS, and every wave has finished reading the
+    // previous stage (its fragments fed MFMAs issued before the barrier), so
+    // that slot takes the stage NSTAGE-1 ahead.
+    auto publish = [&]() {
+        const int freed = pslot;
+        pslot = pslot + 1 == NSTAGE ? 0 : pslot + 1;
+        wait_vmcnt<(NSTAGE - 2) * LPS>();
+        block_barrier();
+        issue(freed);
+    };
+    constexpr int FH = FI / 2;   // A fragments per MFMA half
+    auto read_lo = [&](v4i (&fa)[FH]) {
+        const int8_t *la = lds + pslot * SLOT;
+#pragma unroll
+        for (int i = 0; i < FH; ++i) {
+            const int r = wm * S::WTM + i * 16 + lc;
+            fa[i] = *(const v4i *)(la + r * KT + ((lk ^ swz(r)) << 4));
+        }
+    };
+    auto read_hi = [&](v4i (&fa)[FH], v4i (&fb)[FJ]) {
+        const int8_t *la = lds + pslot * SLOT;
+        const int8_t *lb = la + TILE;
+#pragma unroll
+        for (int i = 0; i < FH; ++i) {
+            const int r = wm * S::WTM + (FH + i) * 16 + lc;
+            fa[i] = *(const v4i *)(la + r * KT + ((lk ^ swz(r)) << 4));
+        }
+#pragma unroll
+        for (int j = 0; j < FJ; ++j) {
+            const int r = wn * S::WTN + j * 16 + lc;
+            fb[j] = *(const v4i *)(lb + r * KT + ((lk ^ swz(r)) << 4));
+        }
+    };
+    auto mfma_half = [&](int i0, const v4i (&fa)[FH], const v4i (&fb)[FJ]) {
+#pragma unroll
+        for (int i = 0; i < FH; ++i)
+#pragma unroll
+            for (int j = 0; j < FJ; ++j)
+                acc[i0 + i][j] =
+                    __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i0 + i][j], 0, 0, 0);
+    };
+    // Software pipeline: the next step is published and its fragments are
+    // read between the two halves of the current step's MFMAs, so barrier
+    // and LDS latency overlap matrix work.  The lower A fragments are read
+    // into the registers the first half has just consumed; upper A and B
+    // fragments alternate between two sets for even and odd steps.
+    v4i alo[FH], ahi0[FH], b0[FJ], ahi1[FH], b1[FJ];
+    auto run_steps = [&](int nsteps) {   // nsteps even
+        publish();
+        read_lo(alo);
+        read_hi(ahi0, b0);
+        for (int n = 0; n < nsteps; n += 2) {
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_half(0, alo, b0);
+            __builtin_amdgcn_sched_barrier(0);
+            publish();
+            read_lo(alo);
+            read_hi(ahi1, b1);
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_half(FH, ahi0, b0);
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_half(0, alo, b1);
+            __builtin_amdgcn_sched_barrier(0);
+            if (n + 2 < nsteps) {
+                publish();
+                read_lo(alo);
+                read_hi(ahi0, b0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_half(FH, ahi1, b1);
+        }
+    };
+
+    if constexpr (!MOD) {
+        bool first_fold = true;
+        for (int t = 2; t <= s + 1; ++t) {
+            const int np = min(s, t - 1) - max(1, t - s) + 1;
+            const int chunk = chunk_tiles(np);
+            for (int c0 = 0; c0 < ktiles; c0 += chunk) {
+#pragma unroll
+                for (int i = 0; i < FI; ++i)
+#pragma unroll
+                    for (int j = 0; j < FJ; ++j) acc[i][j] = vzero;
+                // chunks and KTC are even, so every chunk has an even step count
+                run_steps(min(chunk, ktiles - c0) * np);
+                const int64_t row0 = m0 + wm * S::WTM + 4 * lk;
+                const int64_t col0 = n0 + wn * S::WTN + lc;
+                const bool overwrite = first_fold && !a.accumulate;
+                const int shift = 4 - 8 * t;
+                if (full_tile)
+                    fold_tile<FI, FJ, false>(acc, a.C, a.ldc, a.rexp, a.cexp, M, N, row0, col0,
+                                             shift, overwrite);
+                else
+                    fold_tile<FI, FJ, true>(acc, a.C, a.ldc, a.rexp, a.cexp, M, N, row0, col0,
+                                            shift, overwrite);
+                first_fold = false;
+                // the fold's C traffic retires before the stage counting resumes
+                wait_vmcnt<0>();
+            }
+        }
+    } else {
+        const int64_t tiles = (int64_t)tiles_m * tiles_n;
+        const int64_t tile = (int64_t)tm * tiles_n + tn;
+        for (int t = 0; t < s; ++t) {
+            const int m = (int)MODULI[t];
 #pragma unroll
             for (int i = 0; i < FI; ++i)
 #pragma unroll
                 for (int j = 0; j < FJ; ++j) acc[i][j] = vzero;
-            for (int n = 0; n < nchunk; ++n) {
-                // this wave's loads of the current stage are done once at most
-                // the NSTAGE-2 younger stages are outstanding; its reads of the
-                // previous stage completed before the previous MFMAs
-                wait_vmcnt<(NSTAGE - 2) * LPS>();
-                block_barrier();   // every wave's loads are in LDS; the previous slot is free
-                issue((slot + NSTAGE - 1) & (NSTAGE - 1));
-
-                const int8_t *la = lds + slot * SLOT;
-                const int8_t *lb = la + TILE;
-                v4i af[FI], bf[FJ];
-#pragma unroll
-                for (int i = 0; i < FI; ++i) {
-                    const int r = wm * S::WTM + i * 16 + lc;
-                    af[i] = *(const v4i *)(la + r * KT + ((lk ^ swz(r)) << 4));
-                }
-#pragma unroll
-                for (int j = 0; j < FJ; ++j) {
-                    const int r = wn * S::WTN + j * 16 + lc;
-                    bf[j] = *(const v4i *)(lb + r * KT + ((lk ^ swz(r)) << 4));
-                }
-                // all fragment reads are issued before the first MFMA (the
-                // scheduler would otherwise reuse one register per A fragment
-                // and wait for each read in turn)
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int i = 0; i < FI; ++i)
-#pragma unroll
-                    for (int j = 0; j < FJ; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
-                slot = (slot + 1) & (NSTAGE - 1);
+            for (int c0 = 0; c0 < ktiles; c0 += MOD_CHUNK_TILES) {
+                run_steps(min(MOD_CHUNK_TILES, ktiles - c0));
+                reduce_mod(acc, m);   // keeps the next chunk's INT32 sums in range
             }
-            const int t_cur = t;
-
-        // Fold: each wave dumps its INT32 accumulators to its private LDS
-        // scratch, FOLD_ROWS C rows per pass (pass (i, h): fragment row i, rows
-        // 8h..8h+7, held by the lanes with lk/2 == h), and folds them into C,
-        // one column per lane.  A wave's LDS operations complete in order, so
-        // no workgroup barrier is needed.
-        {
-            const int64_t fcol = n0 + wn * S::WTN + lane;
-            const int ecol = fcol < N ? cexp[fcol] : 0;
 #pragma unroll
-            for (int i = 0; i < FI; ++i) {
+            for (int q = 0; q < S::QPL; ++q) {
+                v4i run;
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    // INT32 layout: lane l, register r holds row 4*(l/16)+r, column l%16.
-                    if ((lk >> 1) == h) {
-#pragma unroll
-                        for (int j = 0; j < FJ; ++j)
-#pragma unroll
-                            for (int r = 0; r < 4; ++r)
-                                fold_buf[((lk & 1) * 4 + r) * FOLD_STR + j * 16 + lc] = acc[i][j][r];
-                    }
-                    asm volatile("" ::: "memory");
-#pragma unroll 2
-                    for (int rho = 0; rho < FOLD_ROWS; ++rho) {
-                        const int64_t row = m0 + wm * S::WTM + i * 16 + h * 8 + rho;
-                        if (row >= M) break;
-                        const int er = rexp[row];
-                        double *c_row = C + row * ldc;
-                        if (fcol < N) {
-                            const int g = fold_buf[rho * FOLD_STR + lane];
-                            const double x = (er == EXP_BAD || ecol == EXP_BAD)
-                                                 ? __longlong_as_double(0x7ff8000000000000ll)
-                                                 : ldexp((double)g, er + ecol + 4 - 8 * t_cur);
-                            c_row[fcol] = (first_fold && !accumulate) ? x : c_row[fcol] + x;
-                        }
-                    }
-                    asm volatile("" ::: "memory");
+                for (int w = 0; w < 4; ++w) {
+                    const int f = 4 * q + w;
+                    const v4i v = acc[f / FJ][f % FJ];
+                    run[w] = (v[0] & 0xff) | ((v[1] & 0xff) << 8) | ((v[2] & 0xff) << 16) |
+                             ((v[3] & 0xff) << 24);
                 }
+                *(v4i *)(a.R + residue_run(t, tiles, tile, S::NW, wave, S::QPL, q) + lane * 16) =
+                    run;
             }
-            first_fold = false;
-            // the fold's C traffic retires before the stage counting resumes
+            // the residue stores retire before the stage counting resumes
             wait_vmcnt<0>();
         }
+    }
+}
+
+// Scheme II reconstruction: one thread per (block tile, wave, lane) of the
+// product kernel's layout; for each of its outputs it gathers the n residues,
+// recovers C' exactly, rounds once and scales.
+template <int TB, int WGM, int WGN>
+__global__ __launch_bounds__(256) void reconstruct_kernel(const uint8_t *__restrict__ R,
+                                                          const int32_t *__restrict__ rexp,
+                                                          const int32_t *__restrict__ cexp,
+                                                          double *__restrict__ C, int64_t M,
+                                                          int64_t N, int64_t ldc, int nmod,
+                                                          int beta, int accumulate) {
+    using S = TileShape<TB, WGM, WGN>;
+    const int tiles_m = (int)((M + TB - 1) / TB);
+    const int tiles_n = (int)((N + TB - 1) / TB);
+    const int64_t tiles = (int64_t)tiles_m * tiles_n;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= tiles * S::THREADS) return;
+    const int64_t tile = gid / S::THREADS;
+    const int wave = (int)((gid / 64) % S::NW);
+    const int lane = (int)(gid % 64);
+    const int64_t m0 = (tile / tiles_n) * TB;
+    const int64_t n0 = (tile % tiles_n) * TB;
+    const int64_t row0 = m0 + (wave / WGN) * S::WTM + 4 * (lane >> 4);
+    const int64_t col0 = n0 + (wave % WGN) * S::WTN + (lane & 15);
+    const int shift = -2 * beta;
+#pragma unroll 1
+    for (int q = 0; q < S::QPL; ++q) {
+        v4i runs[MAX_MODULI];
+#pragma unroll
+        for (int l = 0; l < MAX_MODULI; ++l)
+            if (l < nmod)
+                runs[l] = *(const v4i *)(R + residue_run(l, tiles, tile, S::NW, wave, S::QPL, q) +
+                                         lane * 16);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const int f = 4 * q + w;
+            const int64_t col = col0 + 16 * (f % S::FJ);
+            const int ec = col < N ? cexp[col] : 0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = row0 + 16 * (f / S::FJ) + r;
+                if (row < M && col < N) {
+                    uint32_t res[MAX_MODULI];
+#pragma unroll
+                    for (int l = 0; l < MAX_MODULI; ++l)
+                        res[l] = l < nmod ? ((uint32_t)runs[l][w] >> (8 * r)) & 0xffu : 0u;
+                    const int er = rexp[row];
+                    const double x = (er == EXP_BAD || ec == EXP_BAD)
+                                         ? __longlong_as_double(0x7ff8000000000000ll)
+                                         : ldexp(mgx::modular::reconstruct(res, nmod),
+                                                 er + ec + shift);
+                    double *c = C + row * ldc + col;
+                    *c = accumulate ? *c + x : x;
+                }
+            }
         }
     }
 }
@@ -538,12 +787,18 @@ unsigned grid1d(int64_t total, int threads, int64_t cap = 65536) {
     return (unsigned)b;
 }
 
+bool form_planes_ok(int form, int planes) {
+    if (form == MGX_FORM_INT8) return planes >= MIN_SLICES && planes <= MAX_SLICES;
+    if (form == MGX_FORM_INT8_MOD) return planes >= MIN_MODULI && planes <= MAX_MODULI;
+    return false;
+}
+
 struct SplitLayout {
     int64_t Kp, KTC, Mp, Np;
-    size_t a_off, b_off, rmax_off, cmax_off, rexp_off, cexp_off, total;
+    size_t a_off, b_off, r_off, rmax_off, cmax_off, rexp_off, cexp_off, total;
 };
 
-SplitLayout split_layout(int s, int64_t m, int64_t n, int64_t k) {
+SplitLayout split_layout(int form, int planes, int64_t m, int64_t n, int64_t k) {
     SplitLayout L;
     L.Kp = round_up(k < 1 ? 1 : k, K_ALIGN);
     size_t off = 0;
@@ -555,8 +810,10 @@ SplitLayout split_layout(int s, int64_t m, int64_t n, int64_t k) {
     L.KTC = L.Kp / KT;
     L.Mp = round_up(m < 1 ? 1 : m, ROW_ALIGN);
     L.Np = round_up(n < 1 ? 1 : n, ROW_ALIGN);
-    L.a_off = take((size_t)s * L.Mp * L.Kp);
-    L.b_off = take((size_t)s * L.Np * L.Kp);
+    L.a_off = take((size_t)planes * L.Mp * L.Kp);
+    L.b_off = take((size_t)planes * L.Np * L.Kp);
+    // product residues: every block tile of either tile size fits in Mp x Np
+    L.r_off = take(form == MGX_FORM_INT8_MOD ? (size_t)planes * L.Mp * L.Np : 0);
     L.rmax_off = take((size_t)(m + n) * 8);
     L.cmax_off = L.rmax_off + (size_t)m * 8;
     L.rexp_off = take((size_t)m * 4);
@@ -569,19 +826,29 @@ SplitLayout split_layout(int s, int64_t m, int64_t n, int64_t k) {
 
 extern "C" int mgx_split_slices_range(int form, int *min_slices, int *max_slices) {
     mgx::clear_error();
-    if (form != MGX_FORM_INT8)
+    int lo, hi;
+    if (form == MGX_FORM_INT8) {
+        lo = MIN_SLICES;
+        hi = MAX_SLICES;
+    } else if (form == MGX_FORM_INT8_MOD) {
+        lo = MIN_MODULI;
+        hi = MAX_MODULI;
+    } else {
         return mgx::set_error(MGX_ERR_UNSUPPORTED, "split form not supported");
-    if (min_slices) *min_slices = MIN_SLICES;
-    if (max_slices) *max_slices = MAX_SLICES;
+    }
+    if (min_slices) *min_slices = lo;
+    if (max_slices) *max_slices = hi;
     return MGX_OK;
+}
+
+extern "C" int mgx_split_mod_beta(int moduli, int64_t k) {
+    return mgx::modular::beta_for(moduli, k);
 }
 
 extern "C" size_t mgx_split_workspace_bytes(int form, int slices, int64_t m, int64_t n,
                                             int64_t k) {
-    if (form != MGX_FORM_INT8 || slices < MIN_SLICES || slices > MAX_SLICES || m < 0 || n < 0 ||
-        k < 0)
-        return 0;
-    return split_layout(slices, m, n, k).total;
+    if (!form_planes_ok(form, slices) || m < 0 || n < 0 || k < 0) return 0;
+    return split_layout(form, slices, m, n, k).total;
 }
 
 static int split_gemm_impl(int form, int slices, int tile, const double *A, const double *B,
@@ -593,10 +860,13 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     if (tile != MGX_TILE_LARGE && tile != MGX_TILE_SMALL)
         return mgx::set_error(MGX_ERR_UNSUPPORTED, "unknown tile");
     const int64_t TBsz = tile == MGX_TILE_LARGE ? 256 : 128;
-    if (form != MGX_FORM_INT8)
+    if (form != MGX_FORM_INT8 && form != MGX_FORM_INT8_MOD)
         return mgx::set_error(MGX_ERR_UNSUPPORTED, "split form not supported");
-    if (slices < MIN_SLICES || slices > MAX_SLICES)
-        return mgx::set_error(MGX_ERR_UNSUPPORTED, "slice count out of range");
+    if (!form_planes_ok(form, slices))
+        return mgx::set_error(MGX_ERR_UNSUPPORTED, form == MGX_FORM_INT8
+                                                       ? "slice count out of range"
+                                                       : "modulus count out of range");
+    const bool mod = form == MGX_FORM_INT8_MOD;
     MGX_ARG_CHECK(m >= 0 && n >= 0 && k >= 0, "negative matrix dimension");
     if (m == 0 || n == 0) return MGX_OK;
     MGX_ARG_CHECK(C != nullptr && ldc >= n, "C is NULL or ldc < n");
@@ -609,7 +879,10 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     }
     MGX_ARG_CHECK(A != nullptr && B != nullptr, "A or B is NULL");
     MGX_ARG_CHECK(lda >= k && ldb >= n, "lda < k or ldb < n");
-    const SplitLayout L = split_layout(slices, m, n, k);
+    const int beta = mod ? mgx::modular::beta_for(slices, k) : 0;
+    if (mod && beta < 0)
+        return mgx::set_error(MGX_ERR_UNSUPPORTED, "too few moduli for this k");
+    const SplitLayout L = split_layout(form, slices, m, n, k);
     const int64_t tiles = ((m + TBsz - 1) / TBsz) * ((n + TBsz - 1) / TBsz);
     MGX_ARG_CHECK(tiles <= 0x7fffffffll, "matrix too large for one launch");
 
@@ -624,8 +897,9 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
         ws = (char *)mgx::device_scratch(mgx::SCRATCH_SPLIT, L.total, &status);
         if (!ws) return status;
     }
-    int8_t *Asl = (int8_t *)(ws + L.a_off);
-    int8_t *Bsl = (int8_t *)(ws + L.b_off);
+    int8_t *Aop = (int8_t *)(ws + L.a_off);
+    int8_t *Bop = (int8_t *)(ws + L.b_off);
+    uint8_t *Rres = (uint8_t *)(ws + L.r_off);
     unsigned long long *rmax = (unsigned long long *)(ws + L.rmax_off);
     unsigned long long *cmax = (unsigned long long *)(ws + L.cmax_off);
     int32_t *rexp = (int32_t *)(ws + L.rexp_off);
@@ -661,29 +935,77 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
 
     {
         const int64_t gy = L.Mp / 64;
-        MGX_ARG_CHECK(gy <= 65535 && L.KTC <= 0x7fffffffll, "m or k too large for the row slicer");
-        hipLaunchKernelGGL(slice_rows_kernel, dim3((unsigned)L.KTC, (unsigned)gy), dim3(256), 0, st,
-                           A, m, k, lda, rexp, slices, L.Mp, L.KTC, Asl);
-        if ((rc = mgx::launch_status("slice_rows_kernel")) != MGX_OK) return rc;
+        MGX_ARG_CHECK(gy <= 65535 && L.KTC <= 0x7fffffffll, "m or k too large for the row pass");
+        if (mod)
+            hipLaunchKernelGGL(operand_rows_kernel<true>, dim3((unsigned)L.KTC, (unsigned)gy),
+                               dim3(256), 0, st, A, m, k, lda, rexp, slices, beta, L.Mp, L.KTC,
+                               Aop);
+        else
+            hipLaunchKernelGGL(operand_rows_kernel<false>, dim3((unsigned)L.KTC, (unsigned)gy),
+                               dim3(256), 0, st, A, m, k, lda, rexp, slices, 0, L.Mp, L.KTC, Aop);
+        if ((rc = mgx::launch_status("operand_rows_kernel")) != MGX_OK) return rc;
     }
     {
         const int64_t gy = L.Np / 64;
-        MGX_ARG_CHECK(gy <= 65535, "n too large for the column slicer");
-        hipLaunchKernelGGL(slice_cols_kernel, dim3((unsigned)L.KTC, (unsigned)gy), dim3(256), 0, st,
-                           B, k, n, ldb, cexp, slices, L.Np, L.KTC, Bsl);
-        if ((rc = mgx::launch_status("slice_cols_kernel")) != MGX_OK) return rc;
+        MGX_ARG_CHECK(gy <= 65535, "n too large for the column pass");
+        if (mod)
+            hipLaunchKernelGGL(operand_cols_kernel<true>, dim3((unsigned)L.KTC, (unsigned)gy),
+                               dim3(256), 0, st, B, k, n, ldb, cexp, slices, beta, L.Np, L.KTC,
+                               Bop);
+        else
+            hipLaunchKernelGGL(operand_cols_kernel<false>, dim3((unsigned)L.KTC, (unsigned)gy),
+                               dim3(256), 0, st, B, k, n, ldb, cexp, slices, 0, L.Np, L.KTC, Bop);
+        if ((rc = mgx::launch_status("operand_cols_kernel")) != MGX_OK) return rc;
     }
     if ((rc = mark(2)) != MGX_OK) return rc;
-    if (tile == MGX_TILE_LARGE)
-        hipLaunchKernelGGL((ozaki_int8_gemm<256, 2, 4, 1>), dim3((unsigned)tiles),
-                           dim3(ShapeLarge::THREADS), 0, st, Asl, Bsl, rexp, cexp, C, m, n, L.Mp,
-                           L.Np, L.KTC, ldc, slices, accumulate ? 1 : 0);
-    else
-        hipLaunchKernelGGL((ozaki_int8_gemm<128, 2, 2, 2>), dim3((unsigned)tiles),
-                           dim3(ShapeSmall::THREADS), 0, st, Asl, Bsl, rexp, cexp, C, m, n, L.Mp,
-                           L.Np, L.KTC, ldc, slices, accumulate ? 1 : 0);
-    if ((rc = mgx::launch_status("ozaki_int8_gemm")) != MGX_OK) return rc;
-    return mark(3);
+    ProductArgs pa;
+    pa.A = Aop;
+    pa.B = Bop;
+    pa.rexp = rexp;
+    pa.cexp = cexp;
+    pa.C = C;
+    pa.ldc = ldc;
+    pa.R = Rres;
+    pa.M = m;
+    pa.N = n;
+    pa.Mp = L.Mp;
+    pa.Np = L.Np;
+    pa.KTC = L.KTC;
+    pa.s = slices;
+    pa.accumulate = accumulate ? 1 : 0;
+    if (tile == MGX_TILE_LARGE) {
+        if (mod)
+            hipLaunchKernelGGL((int8_product_kernel<256, 2, 4, true, 1>), dim3((unsigned)tiles),
+                               dim3(ShapeLarge::THREADS), 0, st, pa);
+        else
+            hipLaunchKernelGGL((int8_product_kernel<256, 2, 4, false, 1>), dim3((unsigned)tiles),
+                               dim3(ShapeLarge::THREADS), 0, st, pa);
+    } else {
+        if (mod)
+            hipLaunchKernelGGL((int8_product_kernel<128, 2, 2, true, 2>), dim3((unsigned)tiles),
+                               dim3(ShapeSmall::THREADS), 0, st, pa);
+        else
+            hipLaunchKernelGGL((int8_product_kernel<128, 2, 2, false, 2>), dim3((unsigned)tiles),
+                               dim3(ShapeSmall::THREADS), 0, st, pa);
+    }
+    if ((rc = mgx::launch_status("int8_product_kernel")) != MGX_OK) return rc;
+    if ((rc = mark(3)) != MGX_OK) return rc;
+    if (mod) {
+        const int threads = tile == MGX_TILE_LARGE ? ShapeLarge::THREADS : ShapeSmall::THREADS;
+        const int64_t total = tiles * threads;
+        const int64_t blocks = (total + 255) / 256;
+        MGX_ARG_CHECK(blocks <= 0x7fffffffll, "matrix too large for the reconstruction");
+        if (tile == MGX_TILE_LARGE)
+            hipLaunchKernelGGL((reconstruct_kernel<256, 2, 4>), dim3((unsigned)blocks), dim3(256),
+                               0, st, Rres, rexp, cexp, C, m, n, ldc, slices, beta,
+                               accumulate ? 1 : 0);
+        else
+            hipLaunchKernelGGL((reconstruct_kernel<128, 2, 2>), dim3((unsigned)blocks), dim3(256),
+                               0, st, Rres, rexp, cexp, C, m, n, ldc, slices, beta,
+                               accumulate ? 1 : 0);
+        if ((rc = mgx::launch_status("reconstruct_kernel")) != MGX_OK) return rc;
+    }
+    return MGX_OK;
 }
 
 extern "C" int mgx_split_gemm(int form, int slices, const double *A, const double *B, double *C,

@@ -43,6 +43,10 @@ _SIG = {
     "orc_split_chunk_tiles": (_int, [_int, _int]),
     "orc_split_gemm_int8": (_int, [_int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64,
                                    _int, _int]),
+    "orc_crt_modulus": (_int, [_int]),
+    "orc_crt_log2_product": (_dbl, [_int]),
+    "orc_crt_beta": (_int, [_int, _i64]),
+    "orc_crt_gemm": (_int, [_int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _int, _int]),
     "orc_mlp_param_count": (_i64, [_int]),
     "orc_mlp_forward": (None, [_vp, _int, _vp, _i64, _vp]),
     "orc_mlp_gradient": (_dbl, [_vp, _int, _vp, _vp, _i64, _vp]),
@@ -204,6 +208,35 @@ def split_gemm_int8(A: np.ndarray, B: np.ndarray, s: int, C: np.ndarray | None =
                                    max(n, 1), 1 if acc else 0, nthreads)
     if rc != 0:
         raise RuntimeError(f"orc_split_gemm_int8 failed: {rc}")
+    return out
+
+
+def crt_modulus(l: int) -> int:
+    return int(lib().orc_crt_modulus(l))
+
+
+def crt_log2_product(nmod: int) -> float:
+    return float(lib().orc_crt_log2_product(nmod))
+
+
+def crt_beta(nmod: int, k: int) -> int:
+    """Scaling exponent of the modular form: largest beta <= 53 with 2 k 4^beta < P_n."""
+    return int(lib().orc_crt_beta(nmod, k))
+
+
+def crt_gemm(A: np.ndarray, B: np.ndarray, nmod: int, C: np.ndarray | None = None,
+             nthreads: int = 0) -> np.ndarray:
+    """Bit-exact model of libmgx's INT8 modular product; C given => accumulate form."""
+    m, k = A.shape
+    n = B.shape[1]
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    acc = C is not None
+    out = np.ascontiguousarray(C, dtype=np.float64).copy() if acc else np.empty((m, n))
+    rc = lib().orc_crt_gemm(nmod, _p(A), _p(B), _p(out), m, n, k, max(k, 1), max(n, 1),
+                            max(n, 1), 1 if acc else 0, nthreads)
+    if rc != 0:
+        raise RuntimeError(f"orc_crt_gemm failed: {rc}")
     return out
 
 

@@ -27,6 +27,10 @@
  *  - orc_split_*: a bit-exact model of libmgx's INT8 split-precision
  *    (Ozaki scheme I) product: exponents, slices, exact group sums with the
  *    same INT32 flush chunks, and the same FP64 fold sequence into C.
+ *  - orc_crt_*: a bit-exact model of libmgx's INT8 modular (Ozaki scheme II)
+ *    product: the same exponents, the integer scaling by 2^beta, the exact
+ *    integer product in 128-bit arithmetic and one correctly rounded
+ *    conversion per output.
  *
  * Parity pins (tests/test_oracle.py): exact rational brute force on tiny
  * inputs for the fma chains, the accumulate form, the double-double dot and
@@ -408,6 +412,98 @@ ORC_API int orc_split_gemm_int8(int s, const double *A, const double *B, double 
     }
     free(rexp); free(cexp); free(As); free(Bs);
     return overflow ? -3 : 0;
+}
+
+/* ---- INT8 modular form (Ozaki scheme II) ---------------------------------- */
+/* Published algorithm: K. Ozaki, Y. Uchino, T. Imamura, "Ozaki Scheme II: A
+ * GEMM-oriented emulation of floating-point matrix multiplication using an
+ * integer modular technique" (2025).  A row i is scaled to integers
+ * A'_ik = rint(A_ik 2^(beta-e_i)) (|A'| <= 2^beta), B likewise per column with
+ * f_j; the integer product C' = A'B' is formed exactly (libmgx forms it from
+ * INT8 products modulo n pairwise coprime moduli and Chinese remaindering;
+ * this model forms it directly in 128-bit integers, which is the same integer
+ * whenever |C'| < P/2, P = m_0 ... m_{n-1}, which beta guarantees), and
+ *   C_ij = ldexp((double)C'_ij, e_i + f_j - 2 beta)     (one rounding, RN-even)
+ * (or C_in + that for the accumulate form).  beta is the largest integer in
+ * [0, 53] with 2 k 4^beta < P. */
+#define ORC_MAX_MODULI 16
+static const uint32_t orc_moduli[ORC_MAX_MODULI] = {256, 255, 253, 251, 247, 241, 239, 233,
+                                                     229, 227, 223, 217, 211, 199, 197, 193};
+
+ORC_API int orc_crt_modulus(int l) { return (l >= 0 && l < ORC_MAX_MODULI) ? (int)orc_moduli[l] : -1; }
+
+static unsigned __int128 orc_crt_product(int nmod) {
+    unsigned __int128 P = 1;
+    for (int l = 0; l < nmod; ++l) P *= orc_moduli[l];
+    return P;
+}
+
+/* log2 of P_n (for reporting). */
+ORC_API double orc_crt_log2_product(int nmod) {
+    double s = 0.0;
+    for (int l = 0; l < nmod && l < ORC_MAX_MODULI; ++l) s += log2((double)orc_moduli[l]);
+    return s;
+}
+
+ORC_API int orc_crt_beta(int nmod, int64_t k) {
+    if (nmod < 2 || nmod > ORC_MAX_MODULI || k < 1) return -1;
+    const unsigned __int128 P = orc_crt_product(nmod);
+    unsigned __int128 lhs = (unsigned __int128)2 * (uint64_t)k; /* 2 k 4^beta */
+    if (lhs >= P) return -1;
+    int beta = 0;
+    while (beta < 53 && lhs <= (P - 1) / 4 && lhs * 4 < P) {
+        lhs *= 4;
+        ++beta;
+    }
+    return beta;
+}
+
+ORC_API int orc_crt_gemm(int nmod, const double *A, const double *B, double *C, int64_t m,
+                         int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc,
+                         int accumulate, int nthreads) {
+    set_threads(nthreads);
+    if (m == 0 || n == 0) return 0;
+    if (k == 0) {
+        if (!accumulate)
+            for (int64_t i = 0; i < m; ++i)
+                for (int64_t j = 0; j < n; ++j) C[i * ldc + j] = 0.0;
+        return 0;
+    }
+    const int beta = orc_crt_beta(nmod, k);
+    if (beta < 0) return -1;
+    int32_t *rexp = (int32_t *)malloc(sizeof(int32_t) * m);
+    int32_t *cexp = (int32_t *)malloc(sizeof(int32_t) * n);
+    int64_t *Ai = (int64_t *)malloc(sizeof(int64_t) * m * k);
+    int64_t *Bi = (int64_t *)malloc(sizeof(int64_t) * n * k); /* transposed: [j][kk] */
+    if (!rexp || !cexp || !Ai || !Bi) {
+        free(rexp); free(cexp); free(Ai); free(Bi);
+        return -2;
+    }
+    orc_split_exponents(A, B, m, n, k, lda, ldb, rexp, cexp);
+    for (int64_t i = 0; i < m; ++i)
+        for (int64_t kk = 0; kk < k; ++kk)
+            Ai[i * k + kk] = rexp[i] == ORC_EXP_BAD
+                                 ? 0
+                                 : (int64_t)rint(ldexp(A[i * lda + kk], beta - rexp[i]));
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t kk = 0; kk < k; ++kk)
+            Bi[j * k + kk] = cexp[j] == ORC_EXP_BAD
+                                 ? 0
+                                 : (int64_t)rint(ldexp(B[kk * ldb + j], beta - cexp[j]));
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+    for (int64_t i = 0; i < m; ++i) {
+        for (int64_t j = 0; j < n; ++j) {
+            const int64_t *ar = Ai + i * k;
+            const int64_t *br = Bi + j * k;
+            __int128 acc = 0;
+            for (int64_t kk = 0; kk < k; ++kk) acc += (__int128)ar[kk] * br[kk];
+            const int bad = rexp[i] == ORC_EXP_BAD || cexp[j] == ORC_EXP_BAD;
+            const double x = bad ? NAN : ldexp((double)acc, rexp[i] + cexp[j] - 2 * beta);
+            C[i * ldc + j] = accumulate ? C[i * ldc + j] + x : x;
+        }
+    }
+    free(rexp); free(cexp); free(Ai); free(Bi);
+    return 0;
 }
 
 /* ---- cost-model MLP (double precision) ------------------------------------ */
