@@ -5,18 +5,22 @@ vs native FP64 MFMA, on one MI355X.
 Workload (one step): one rank's local product of the 2D SUMMA 2x2 grid at
 N=16384, C_ij (8192 x 8192) = A_ik (8192 x 16384) . B_kj (16384 x 8192), FP64,
 inputs uniform[-1,1) from the splitmix64 stream (A seed 1, B seed 2), all
-operands resident in HBM.  Both forms are credited F = 2 M N K flop.
+operands resident in HBM.  Every form is credited F = 2 M N K flop.
 
-  * split (the `value`): libmgx INT8 Ozaki product with `--slices` slices,
-    timed end to end (exponent scan, slicing, all slice-pair products,
-    reconstruction);
+  * split (the `value`): libmgx INT8 modular product (Ozaki scheme II) with
+    `--moduli` moduli, timed end to end (exponent scan, residue extraction,
+    the per-modulus INT8 products, exact reconstruction into C);
+  * split_slices: libmgx INT8 digit-slice product (Ozaki scheme I) with
+    `--slices` slices, timed the same way;
   * plain: libmgx FP64 MFMA product on the same inputs.
 
-The dominant kernel of the split form is the fused slice-pair product; its
-duration is read from HIP events recorded on the launch stream around that
-kernel inside the timed region.  The CPU baseline is the oracle's
-restatement of the reference kernel's arithmetic (sequential fma chains)
-run with OpenMP on the host cores over a row slab of the same product.
+Each split form's error against a double-double A.B on sampled rows is
+reported next to the plain kernel's.  The dominant kernel of the headline
+form is the INT8 residue-product kernel; its duration is read from HIP
+events recorded on the launch stream around it inside the timed region.  The
+CPU baseline is the oracle's restatement of the reference kernel's
+arithmetic (sequential fma chains) run with OpenMP on the host cores over a
+row slab of the same product.
 
 Prints ONE JSON line on stdout.
 """
@@ -36,7 +40,10 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--slices", type=int, default=7)
+    ap.add_argument("--moduli", type=int, default=15,
+                    help="moduli of the headline INT8 modular form")
+    ap.add_argument("--slices", type=int, default=7,
+                    help="slices of the INT8 digit-slice form (secondary line)")
     ap.add_argument("--m", type=int, default=8192)
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--k", type=int, default=16384)
@@ -85,7 +92,7 @@ def cpu_baseline(m: int, n: int, k: int, target_s: float) -> dict:
     }
 
 
-def summa_report(mgx, torch, A, B, m, n, k, s) -> dict:
+def summa_report(mgx, torch, A, B, m, n, k, nmod) -> dict:
     """The same rank share with panels streamed from pinned host strips
     (hipMemcpy2DAsync on a transfer stream, hipGraph-captured panel loop),
     against the identical panel loop on HBM-resident operands."""
@@ -97,10 +104,10 @@ def summa_report(mgx, torch, A, B, m, n, k, s) -> dict:
     hB.copy_(B)
     torch.cuda.synchronize()
     dC = torch.empty((m, n), dtype=torch.float64, device=A.device)
-    rep = {"form": f"int8 s={s}", "h2d_bytes": 8 * (m * k + k * n), "panels": {}}
+    rep = {"form": f"int8 modular, {nmod} moduli", "h2d_bytes": 8 * (m * k + k * n), "panels": {}}
     for panel in (256, 512, 1024):
         row = {}
-        for form_name, form, sl in (("split", mgx.FORM_INT8, s), ("plain", mgx.FORM_FP64, 0)):
+        for form_name, form, sl in (("split", mgx.FORM_INT8_MOD, nmod), ("plain", mgx.FORM_FP64, 0)):
             dC.zero_()
             torch.cuda.synchronize()
             st = mgx.summa_run(hA.data_ptr(), k, hB.data_ptr(), n, dC, n, m=m, n=n, k=k,
@@ -133,17 +140,18 @@ def main() -> int:
     import torch
     import mgxbench_amd as mgx
 
-    m, n, k, s = args.m, args.n, args.k, args.slices
+    m, n, k = args.m, args.n, args.k
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    log(f"allocating m={m} n={n} k={k} slices={s}")
+    log(f"allocating m={m} n={n} k={k} moduli={args.moduli} slices={args.slices}")
     A = torch.empty((m, k), dtype=torch.float64, device=dev)
     B = torch.empty((k, n), dtype=torch.float64, device=dev)
     C = torch.empty((m, n), dtype=torch.float64, device=dev)
     Cp = torch.empty((m, n), dtype=torch.float64, device=dev)
     mgx.fill(A, m, k, k, 1, seed=1)
     mgx.fill(B, k, n, n, 1, seed=2)
-    ws_bytes = mgx.split_workspace_bytes(mgx.FORM_INT8, s, m, n, k)
+    forms = {"mod": (mgx.FORM_INT8_MOD, args.moduli), "slices": (mgx.FORM_INT8, args.slices)}
+    ws_bytes = max(mgx.split_workspace_bytes(f, p, m, n, k) for f, p in forms.values())
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
     F = 2.0 * m * n * k
@@ -154,17 +162,6 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
         del src, dst
-
-    def plain_step():
-        mgx.ABMultiply(A, B, Cp, m, n, k, k, n, n, 32, stream=stream)
-
-    def split_step(ev=None):
-        if ev is None:
-            mgx.split_gemm(A, B, C, m, n, k, k, n, n, slices=s, workspace=ws,
-                           workspace_bytes=ws_bytes, stream=stream)
-        else:
-            mgx.split_gemm_events(A, B, C, m, n, k, k, n, n, ev, slices=s, workspace=ws,
-                                  workspace_bytes=ws_bytes, stream=stream)
 
     def timed(step, steps):
         t0 = torch.cuda.Event(enable_timing=True)
@@ -180,47 +177,67 @@ def main() -> int:
     # plain FP64 MFMA
     log("plain warmup")
     for _ in range(args.warmup):
-        plain_step()
+        mgx.ABMultiply(A, B, Cp, m, n, k, k, n, n, 32, stream=stream)
     torch.cuda.synchronize()
     log("plain timed")
-    ms_plain = timed(lambda i: plain_step(), args.steps)
+    ms_plain = timed(lambda i: mgx.ABMultiply(A, B, Cp, m, n, k, k, n, n, 32, stream=stream),
+                     args.steps)
     log(f"plain {ms_plain:.3f} ms/step")
-
-    # split precision, with phase events inside the timed region
-    events = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
-    for evs in events:  # materialise the HIP events before the timed region
-        for e in evs:
-            e.record(stream)
-    log("split warmup")
-    for _ in range(args.warmup):
-        split_step()
-    torch.cuda.synchronize()
-    log("split timed")
-    ms_split = timed(lambda i: split_step(events[i]), args.steps)
-    gemm_ms = sum(ev[2].elapsed_time(ev[3]) for ev in events) / args.steps
-    scan_ms = sum(ev[0].elapsed_time(ev[1]) for ev in events) / args.steps
-    slice_ms = sum(ev[1].elapsed_time(ev[2]) for ev in events) / args.steps
-    log(f"split {ms_split:.3f} ms/step (scan {scan_ms:.3f}, slice {slice_ms:.3f}, "
-        f"product {gemm_ms:.3f})")
-
-    # agreement of the two forms on the full share (both FP64-accurate)
-    diff = float(torch.linalg.vector_norm(C - Cp) / torch.linalg.vector_norm(Cp))
-    # error of each form against a double-double A.B on evenly sampled rows
-    log("accuracy check (double-double, sampled rows)")
     rows = list(range(0, m, max(1, m // 16)))[:16]
-    acc = {"rows_sampled": len(rows),
-           "split": mgx.error_report(A, B, C, m, n, k, k, n, n, rows),
-           "plain": mgx.error_report(A, B, Cp, m, n, k, k, n, n, rows)}
-    for form in ("split", "plain"):
-        acc[form].pop("rows_checked", None)
-    log(f"normwise error split {acc['split']['normwise']:.3e} plain {acc['plain']['normwise']:.3e}")
+    acc_plain = mgx.error_report(A, B, Cp, m, n, k, k, n, n, rows)
+    acc_plain.pop("rows_checked", None)
 
-    tf_split = F / (ms_split * 1e-3) / 1e12
+    def measure(name):
+        form, planes = forms[name]
+        # phase events [scan, operands, product, (reconstruction) end] inside the timed region
+        events = [[torch.cuda.Event(enable_timing=True) for _ in range(5)]
+                  for _ in range(args.steps)]
+        for evs in events:  # materialise the HIP events before the timed region
+            for e in evs:
+                e.record(stream)
+
+        def step(i=None):
+            if i is None:
+                mgx.split_gemm(A, B, C, m, n, k, k, n, n, form=form, slices=planes, workspace=ws,
+                               workspace_bytes=ws_bytes, stream=stream)
+            else:
+                mgx.split_gemm_events(A, B, C, m, n, k, k, n, n, events[i][:4], form=form,
+                                      slices=planes, workspace=ws, workspace_bytes=ws_bytes,
+                                      stream=stream)
+                events[i][4].record(stream)
+
+        log(f"{name} warmup")
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        log(f"{name} timed")
+        ms = timed(step, args.steps)
+
+        def avg(a, b):
+            return sum(ev[a].elapsed_time(ev[b]) for ev in events) / args.steps
+
+        phases = {"exponent_scan": avg(0, 1), "operands": avg(1, 2), "int8_products": avg(2, 3),
+                  "reconstruction": avg(3, 4)}
+        log(f"{name} {ms:.3f} ms/step " + " ".join(f"{p} {v:.3f}" for p, v in phases.items()))
+        diff = float(torch.linalg.vector_norm(C - Cp) / torch.linalg.vector_norm(Cp))
+        acc = mgx.error_report(A, B, C, m, n, k, k, n, n, rows)
+        acc.pop("rows_checked", None)
+        products = planes if name == "mod" else planes * (planes + 1) // 2
+        return {"ms": ms, "phases": phases, "diff": diff, "acc": acc, "products": products,
+                "planes": planes}
+
+    res = {name: measure(name) for name in ("mod", "slices")}
+    head = res["mod"]
+    log(f"normwise error modular {head['acc']['normwise']:.3e} slices "
+        f"{res['slices']['acc']['normwise']:.3e} plain {acc_plain['normwise']:.3e}")
+
     tf_plain = F / (ms_plain * 1e-3) / 1e12
-    pairs = s * (s + 1) // 2
-    int8_ops = pairs * F
-    achieved = int8_ops / (gemm_ms * 1e-3) / 1e12
+    tf_split = F / (head["ms"] * 1e-3) / 1e12
+    int8_ops = head["products"] * F
+    achieved = int8_ops / (head["phases"]["int8_products"] * 1e-3) / 1e12
     peak_i8 = mgx.peak(mgx.FORM_INT8)
+    beta = mgx._lib.lib().mgx_split_mod_beta(args.moduli, k)
+    sl = res["slices"]
     out = {
         "metric": METRIC,
         "value": round(tf_split, 3),
@@ -228,7 +245,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_split, 4),
+        "ms_per_step": round(head["ms"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -238,7 +255,8 @@ def main() -> int:
             "workload": (f"2D SUMMA 2x2 rank share at N=16384: C {m}x{n} += A {m}x{k} . "
                          f"B {k}x{n}, FP64 uniform[-1,1), HBM-resident"),
             "m": m, "n": n, "k": k,
-            "form": f"int8 Ozaki split, s={s} slices ({pairs} slice-pair products)",
+            "form": (f"int8 modular split (Ozaki scheme II), {args.moduli} moduli, "
+                     f"beta={beta} ({args.moduli} INT8 products)"),
             "parallelism": "single GPU",
         },
         "plain_fp64": {
@@ -247,13 +265,22 @@ def main() -> int:
             "frac_of_fp64_mfma_peak": round(tf_plain / mgx.peak(mgx.FORM_FP64), 4),
         },
         "speedup_vs_plain": round(tf_split / tf_plain, 4),
-        "split_phase_ms": {"exponent_scan": round(scan_ms, 4), "slicing": round(slice_ms, 4),
-                           "slice_pair_product": round(gemm_ms, 4)},
-        "split_vs_plain_rel_fro": diff,
-        "accuracy_vs_double_double": acc,
-        "split_no_worse_than_plain": acc["split"]["normwise"] <= acc["plain"]["normwise"],
+        "split_phase_ms": {p: round(v, 4) for p, v in head["phases"].items()},
+        "split_vs_plain_rel_fro": head["diff"],
+        "accuracy_vs_double_double": {"rows_sampled": len(rows), "split": head["acc"],
+                                      "plain": acc_plain, "split_slices": sl["acc"]},
+        "split_no_worse_than_plain": head["acc"]["normwise"] <= acc_plain["normwise"],
+        "split_slices": {
+            "form": (f"int8 digit-slice split (Ozaki scheme I), s={sl['planes']} "
+                     f"({sl['products']} slice-pair products)"),
+            "tflops": round(F / (sl["ms"] * 1e-3) / 1e12, 3),
+            "ms_per_step": round(sl["ms"], 4),
+            "speedup_vs_plain": round(ms_plain / sl["ms"], 4),
+            "phase_ms": {p: round(v, 4) for p, v in sl["phases"].items() if p != "reconstruction"},
+            "no_worse_than_plain": sl["acc"]["normwise"] <= acc_plain["normwise"],
+        },
         "roofline": {
-            "kernel": "ozaki_int8_gemm (fused slice-pair product)",
+            "kernel": "int8_product_kernel (per-modulus INT8 residue products)",
             "bound": "mfma",
             "achieved": round(achieved, 2),
             "peak": peak_i8,
@@ -263,7 +290,7 @@ def main() -> int:
         },
     }
     if not args.no_summa:
-        out["summa_streamed"] = summa_report(mgx, torch, A, B, m, n, k, s)
+        out["summa_streamed"] = summa_report(mgx, torch, A, B, m, n, k, args.moduli)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(m, n, k, args.cpu_seconds)
