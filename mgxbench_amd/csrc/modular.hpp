@@ -1,5 +1,5 @@
 // Modular arithmetic of the INT8 modular split form (Ozaki scheme II):
-// the moduli, Garner's mixed-radix tables, residue extraction and the exact
+// the moduli, the Chinese-remainder tables, residue extraction and the exact
 // reconstruction of each output.  Internal to libmgx.
 //
 // Published algorithm: K. Ozaki, Y. Uchino, T. Imamura, "Ozaki Scheme II: A
@@ -25,17 +25,6 @@ constexpr int MAX_MODULI = 16;
 constexpr uint32_t MODULI[MAX_MODULI] = {256, 255, 253, 251, 247, 241, 239, 233,
                                          229, 227, 223, 217, 211, 199, 197, 193};
 
-// Garner tables for the prefix of the moduli list.  Mixed radix:
-//   x = v_0 + v_1 m_0 + v_2 m_0 m_1 + ...,  0 <= v_l < m_l,
-//   v_l = ((r_l - sum_{j<l} v_j PM[l][j]) INV[l]) mod m_l,
-// with PM[l][j] = (m_0 ... m_{j-1}) mod m_l and INV[l] the inverse of
-// (m_0 ... m_{l-1}) mod m_l.  PREFIX[l] = m_0 ... m_{l-1} in four 32-bit limbs.
-struct GarnerTables {
-    uint32_t pm[MAX_MODULI][MAX_MODULI];
-    uint32_t inv[MAX_MODULI];
-    uint32_t prefix[MAX_MODULI + 1][4];
-};
-
 constexpr uint32_t inverse_mod(uint32_t a, uint32_t m) {
     // extended Euclid on (a mod m, m); a and m coprime
     int64_t r0 = m, r1 = a % m, t0 = 0, t1 = 1;
@@ -50,32 +39,6 @@ constexpr uint32_t inverse_mod(uint32_t a, uint32_t m) {
     }
     return (uint32_t)(((t0 % (int64_t)m) + m) % m);
 }
-
-constexpr GarnerTables make_garner_tables() {
-    GarnerTables T{};
-    for (int l = 0; l < MAX_MODULI; ++l) {
-        uint32_t prod = 1 % MODULI[l];
-        for (int j = 0; j < MAX_MODULI; ++j) {
-            T.pm[l][j] = j < l ? prod : 0;
-            if (j < l) prod = (uint32_t)((uint64_t)prod * MODULI[j] % MODULI[l]);
-        }
-        T.inv[l] = l == 0 ? 1 : inverse_mod(prod, MODULI[l]);
-    }
-    uint32_t limb[4] = {1, 0, 0, 0};
-    for (int l = 0; l <= MAX_MODULI; ++l) {
-        for (int w = 0; w < 4; ++w) T.prefix[l][w] = limb[w];
-        if (l == MAX_MODULI) break;
-        uint64_t carry = 0;
-        for (int w = 0; w < 4; ++w) {
-            const uint64_t v = (uint64_t)limb[w] * MODULI[l] + carry;
-            limb[w] = (uint32_t)v;
-            carry = v >> 32;
-        }
-    }
-    return T;
-}
-
-constexpr GarnerTables GARNER = make_garner_tables();
 
 // Scaling exponent for n moduli and depth k: the largest beta in [0, 53] with
 // 2 k 4^beta < P_n, or -1 when even beta = 0 does not fit.
@@ -131,71 +94,127 @@ __host__ __device__ inline double u128_to_double_rn(uint64_t hi, uint64_t lo) {
     return ldexp((double)mant, e);
 }
 
-// Exact integer x with x = r_l (mod m_l), |x| < P/2, from residues
-// r_l in [0, m_l), l < n, rounded to double.
-__host__ __device__ inline double reconstruct(const uint32_t (&r)[MAX_MODULI], int n) {
-    uint32_t v[MAX_MODULI];
-    v[0] = r[0];
-#pragma unroll
-    for (int l = 1; l < MAX_MODULI; ++l) {
-        if (l < n) {
-            uint32_t s = 0;   // < 15 * 255 * 255 < 2^20
-#pragma unroll
-            for (int j = 0; j < l; ++j) s += v[j] * GARNER.pm[l][j];
-            const uint32_t m = MODULI[l];
-            const uint32_t t = s % m;
-            const uint32_t d = r[l] >= t ? r[l] - t : r[l] + m - t;
-            v[l] = d * GARNER.inv[l] % m;
+// Direct Chinese remaindering tables for n = 2..16 moduli:
+//   x = (sum_l r_l W_l) mod P_n,   W_l = (P_n / m_l) ((P_n / m_l)^-1 mod m_l) < P_n,
+// with W and P_n in eight 16-bit limbs and 1/P_n as a double.
+struct CrtTables {
+    uint32_t w[MAX_MODULI + 1][MAX_MODULI][8];
+    uint32_t p[MAX_MODULI + 1][8];
+    double inv_p[MAX_MODULI + 1];
+};
+
+constexpr CrtTables make_crt_tables() {
+    CrtTables T{};
+    for (int n = MIN_MODULI; n <= MAX_MODULI; ++n) {
+        unsigned __int128 P = 1;
+        for (int l = 0; l < n; ++l) P *= MODULI[l];
+        for (int w = 0; w < 8; ++w) T.p[n][w] = (uint32_t)(P >> (16 * w)) & 0xffffu;
+        // 1/P rounded twice (conversion of P, division); only used for an estimate
+        T.inv_p[n] = 1.0 / (double)P;
+        for (int l = 0; l < n; ++l) {
+            const unsigned __int128 Ml = P / MODULI[l];
+            const uint32_t y = inverse_mod((uint32_t)(Ml % MODULI[l]), MODULI[l]);
+            const unsigned __int128 W = Ml * y;
+            for (int w = 0; w < 8; ++w) T.w[n][l][w] = (uint32_t)(W >> (16 * w)) & 0xffffu;
         }
     }
-    // x = sum_l v_l PREFIX[l] in 32-bit limbs (each limb sum < 16 * 2^40)
-    uint64_t acc[4] = {0, 0, 0, 0};
+    return T;
+}
+
+// Exact integer x with x = r_l (mod m_l), |x| < P/2, from residues
+// r_l in [0, m_l), l < n, rounded to double (T = make_crt_tables()).
+//   S = sum_l r_l W_l       (16-bit limbs, each limb sum < 16 * 2^24)
+//   q = floor(S / P) from a double estimate, corrected by one either way
+//   x = S - q P in [0, P);  C' = x, or x - P when x > P / 2.
+__host__ __device__ inline double reconstruct(const CrtTables &T, const uint32_t (&r)[MAX_MODULI],
+                                              int n) {
+    const uint32_t(&W)[MAX_MODULI][8] = T.w[n];
+    const uint32_t(&P)[8] = T.p[n];
+    uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int l = 0; l < MAX_MODULI; ++l) {
-        if (l < n) {
+        const uint32_t rl = l < n ? (r[l] & 0xffu) : 0u;
 #pragma unroll
-            for (int w = 0; w < 4; ++w) acc[w] += (uint64_t)v[l] * GARNER.prefix[l][w];
+        for (int w = 0; w < 8; ++w) acc[w] += rl * (W[l][w] & 0xffffu);
+    }
+    // normalise to 16-bit limbs s[0..8] (S < 16 * 256 * P < 2^139)
+    uint32_t sl[9];
+    uint32_t carry = 0;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+        const uint32_t t = acc[w] + carry;
+        sl[w] = t & 0xffffu;
+        carry = t >> 16;
+    }
+    sl[8] = carry;
+    double sd = 0.0;
+#pragma unroll
+    for (int w = 8; w >= 0; --w) sd = sd * 65536.0 + (double)sl[w];
+    int32_t q = (int32_t)floor(sd * T.inv_p[n]);   // 0 <= q < 4096, off by at most one
+    // x = S - q P (signed 16-bit limb arithmetic with borrow)
+    int32_t xl[9];
+    int32_t borrow = 0;
+#pragma unroll
+    for (int w = 0; w < 9; ++w) {
+        const int32_t pw = w < 8 ? (int32_t)(P[w] & 0xffffu) : 0;
+        int32_t t = (int32_t)sl[w] - q * pw - borrow;   // |q * pw| < 2^28
+        borrow = 0;
+        // floor division of t by 2^16 (arithmetic shift) -> next borrow
+        borrow = -(t >> 16);
+        xl[w] = t & 0xffff;
+    }
+    // the value is sum xl[w] 2^(16w) - borrow 2^144; borrow != 0 means x < 0
+    auto add_p = [&](int sign) {
+        int32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < 9; ++w) {
+            const int32_t pw = w < 8 ? (int32_t)(P[w] & 0xffffu) : 0;
+            const int32_t t = xl[w] + sign * pw + c;
+            c = t >> 16;
+            xl[w] = t & 0xffff;
         }
-    }
-    uint32_t x[4];
-    uint64_t carry = 0;
+        return c;
+    };
+    if (borrow != 0) borrow -= add_p(1);   // q was one too large
+    // x >= P ?  compare limbs from the top
+    {
+        bool ge = true, decided = false;
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
-        const uint64_t t = acc[w] + carry;
-        x[w] = (uint32_t)t;
-        carry = t >> 32;
+        for (int w = 8; w >= 0; --w) {
+            const uint32_t pw = w < 8 ? (P[w] & 0xffffu) : 0u;
+            if (!decided && (uint32_t)xl[w] != pw) {
+                ge = (uint32_t)xl[w] > pw;
+                decided = true;
+            }
+        }
+        if (ge) add_p(-1);   // q was one too small
     }
-    // P_n and the half-range test x >= P - x  <=>  x > P/2 (P_n is even: m_0 = 256)
-    uint32_t P[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-        uint32_t pw = 0;
-#pragma unroll
-        for (int l = 1; l <= MAX_MODULI; ++l)
-            if (l == n) pw = GARNER.prefix[l][w];
-        P[w] = pw;
-    }
-    // y = P - x
+    // symmetric range: x > P/2  <=>  2x > P
+    uint32_t x32[4] = {(uint32_t)xl[0] | ((uint32_t)xl[1] << 16),
+                       (uint32_t)xl[2] | ((uint32_t)xl[3] << 16),
+                       (uint32_t)xl[4] | ((uint32_t)xl[5] << 16),
+                       (uint32_t)xl[6] | ((uint32_t)xl[7] << 16)};
+    uint32_t p32[4] = {(P[0] & 0xffffu) | (P[1] << 16), (P[2] & 0xffffu) | (P[3] << 16),
+                       (P[4] & 0xffffu) | (P[5] << 16), (P[6] & 0xffffu) | (P[7] << 16)};
     uint32_t y[4];
-    int64_t borrow = 0;
+    int64_t bo = 0;
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
-        const int64_t t = (int64_t)P[w] - (int64_t)x[w] - borrow;
+        const int64_t t = (int64_t)p32[w] - (int64_t)x32[w] - bo;
         y[w] = (uint32_t)t;
-        borrow = t < 0 ? 1 : 0;
+        bo = t < 0 ? 1 : 0;
     }
-    // x > P/2  <=>  x > y
     bool greater = false, decided = false;
 #pragma unroll
     for (int w = 3; w >= 0; --w) {
-        if (!decided && x[w] != y[w]) {
-            greater = x[w] > y[w];
+        if (!decided && x32[w] != y[w]) {
+            greater = x32[w] > y[w];
             decided = true;
         }
     }
     uint32_t mag[4];
 #pragma unroll
-    for (int w = 0; w < 4; ++w) mag[w] = greater ? y[w] : x[w];
+    for (int w = 0; w < 4; ++w) mag[w] = greater ? y[w] : x32[w];
     const double d = u128_to_double_rn(((uint64_t)mag[3] << 32) | mag[2],
                                        ((uint64_t)mag[1] << 32) | mag[0]);
     return greater ? -d : d;

@@ -60,6 +60,9 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 inline int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
+// Chinese-remainder tables of the modular form (n = 2..16), in constant memory.
+__constant__ mgx::modular::CrtTables CRT_TABLES = mgx::modular::make_crt_tables();
+
 // ---- exponent scan --------------------------------------------------------
 
 __global__ void zero_u64_kernel(unsigned long long *p, int64_t n) {
@@ -212,58 +215,56 @@ __device__ inline double scaled_operand(double a, int e, int beta) {
     return MOD ? rint(ldexp(a, beta - e)) : ldexp(a, -e);
 }
 
-// One workgroup converts 64 rows x 64 k of A: lane t takes row t/4 and the 16
-// k values (t%4)*16.. of it, so 64 lanes write one contiguous KiB per plane.
-template <bool MOD>
-__global__ __launch_bounds__(256) void operand_rows_kernel(const double *__restrict__ A,
-                                                           int64_t m, int64_t k, int64_t lda,
-                                                           const int32_t *__restrict__ rexp,
-                                                           int planes, int beta, int64_t Mp,
-                                                           int64_t KTC, int8_t *__restrict__ Aop) {
+// One workgroup converts a 64 x 64 tile of operand lines (rows of A or
+// columns of B) by 64 k.  The tile is first staged in LDS by coalesced loads
+// (a wave reads one 512-byte row of the source matrix per instruction: a row
+// of A is a line, a row of B crosses 64 lines), then lane t of wave w
+// converts line 16 w + t % 16, k values 16 (t / 16) .. +15, and writes 16
+// bytes per plane; the 64 lanes of a wave write one contiguous KiB per plane.
+// LDS rows are padded to 528 bytes so that both read patterns are free of
+// bank conflicts.
+template <bool MOD, bool COLS>
+__global__ __launch_bounds__(256) void operand_tile_kernel(const double *__restrict__ X,
+                                                           int64_t lines, int64_t k, int64_t ld,
+                                                           const int32_t *__restrict__ ex,
+                                                           int planes, int beta, int64_t Lp,
+                                                           int64_t KTC, int8_t *__restrict__ Xop) {
+    constexpr int STR = 66;   // doubles per padded LDS row
+    __shared__ double tile[64 * STR];
     const int64_t kt = blockIdx.x;
-    const int64_t row = (int64_t)blockIdx.y * 64 + (threadIdx.x >> 2);
-    const int kq = threadIdx.x & 3;
-    const int64_t k0 = kt * KT + kq * 16;
-    const int e = row < m ? rexp[row] : EXP_BAD;
+    const int64_t l0 = (int64_t)blockIdx.y * 64;
+    const int64_t k0 = kt * KT;
+    const int t = threadIdx.x;
+    // stage: element e = t + 256 i is (source row e / 64, source column e % 64)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int e = t + 256 * i;
+        const int sr = e >> 6, sc = e & 63;
+        double v = 0.0;
+        if (!COLS) {   // source row = line, source column = k
+            if (l0 + sr < lines && k0 + sc < k) v = X[(l0 + sr) * ld + k0 + sc];
+        } else {       // source row = k, source column = line
+            if (k0 + sr < k && l0 + sc < lines) v = X[(k0 + sr) * ld + l0 + sc];
+        }
+        tile[sr * STR + sc] = v;
+    }
+    __syncthreads();
+    const int lane = t & 63;
+    const int line = (t >> 6) * 16 + (lane & 15);
+    const int kq = lane >> 4;
+    const int64_t gl = l0 + line;
+    const int e = gl < lines ? ex[gl] : EXP_BAD;
     double x[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
-        const int64_t kk = k0 + i;
-        x[i] = (row < m && kk < k && e != EXP_BAD) ? scaled_operand<MOD>(A[row * lda + kk], e, beta)
-                                                   : 0.0;
+        const double a = COLS ? tile[(kq * 16 + i) * STR + line] : tile[line * STR + kq * 16 + i];
+        x[i] = e != EXP_BAD ? scaled_operand<MOD>(a, e, beta) : 0.0;
     }
-    int8_t *dst = Aop + (kt * Mp + row) * KT + kq * 16;
+    int8_t *dst = Xop + (kt * Lp + gl) * KT + kq * 16;
     if (MOD)
-        emit_residues16(x, planes, dst, KTC * Mp * KT);
+        emit_residues16(x, planes, dst, KTC * Lp * KT);
     else
-        emit_slices16(x, planes, dst, KTC * Mp * KT);
-}
-
-// One workgroup converts 64 columns x 64 k of B: lane t takes column t/4 and
-// the 16 k values (t%4)*16.. of it.
-template <bool MOD>
-__global__ __launch_bounds__(256) void operand_cols_kernel(const double *__restrict__ B,
-                                                           int64_t k, int64_t n, int64_t ldb,
-                                                           const int32_t *__restrict__ cexp,
-                                                           int planes, int beta, int64_t Np,
-                                                           int64_t KTC, int8_t *__restrict__ Bop) {
-    const int64_t kt = blockIdx.x;
-    const int64_t c = (int64_t)blockIdx.y * 64 + (threadIdx.x >> 2);
-    const int kq = threadIdx.x & 3;
-    const int64_t k0 = kt * KT + kq * 16;
-    const int e = c < n ? cexp[c] : EXP_BAD;
-    double x[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int64_t kk = k0 + i;
-        x[i] = (c < n && kk < k && e != EXP_BAD) ? scaled_operand<MOD>(B[kk * ldb + c], e, beta)
-                                                 : 0.0;
-    }
-    int8_t *dst = Bop + (kt * Np + c) * KT + kq * 16;
-    if (MOD)
-        emit_residues16(x, planes, dst, KTC * Np * KT);
-    else
-        emit_slices16(x, planes, dst, KTC * Np * KT);
+        emit_slices16(x, planes, dst, KTC * Lp * KT);
 }
 
 // ---- INT8 product kernel ------------------------------------------------------
@@ -761,7 +762,7 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(const uint8_t *__restr
                     const int er = rexp[row];
                     const double x = (er == EXP_BAD || ec == EXP_BAD)
                                          ? __longlong_as_double(0x7ff8000000000000ll)
-                                         : ldexp(mgx::modular::reconstruct(res, nmod),
+                                         : ldexp(mgx::modular::reconstruct(CRT_TABLES, res, nmod),
                                                  er + ec + shift);
                     double *c = C + row * ldc + col;
                     *c = accumulate ? *c + x : x;
@@ -936,26 +937,26 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     {
         const int64_t gy = L.Mp / 64;
         MGX_ARG_CHECK(gy <= 65535 && L.KTC <= 0x7fffffffll, "m or k too large for the row pass");
+        const dim3 grid((unsigned)L.KTC, (unsigned)gy);
         if (mod)
-            hipLaunchKernelGGL(operand_rows_kernel<true>, dim3((unsigned)L.KTC, (unsigned)gy),
-                               dim3(256), 0, st, A, m, k, lda, rexp, slices, beta, L.Mp, L.KTC,
-                               Aop);
+            hipLaunchKernelGGL((operand_tile_kernel<true, false>), grid, dim3(256), 0, st, A, m, k,
+                               lda, rexp, slices, beta, L.Mp, L.KTC, Aop);
         else
-            hipLaunchKernelGGL(operand_rows_kernel<false>, dim3((unsigned)L.KTC, (unsigned)gy),
-                               dim3(256), 0, st, A, m, k, lda, rexp, slices, 0, L.Mp, L.KTC, Aop);
-        if ((rc = mgx::launch_status("operand_rows_kernel")) != MGX_OK) return rc;
+            hipLaunchKernelGGL((operand_tile_kernel<false, false>), grid, dim3(256), 0, st, A, m, k,
+                               lda, rexp, slices, 0, L.Mp, L.KTC, Aop);
+        if ((rc = mgx::launch_status("operand_tile_kernel (A)")) != MGX_OK) return rc;
     }
     {
         const int64_t gy = L.Np / 64;
         MGX_ARG_CHECK(gy <= 65535, "n too large for the column pass");
+        const dim3 grid((unsigned)L.KTC, (unsigned)gy);
         if (mod)
-            hipLaunchKernelGGL(operand_cols_kernel<true>, dim3((unsigned)L.KTC, (unsigned)gy),
-                               dim3(256), 0, st, B, k, n, ldb, cexp, slices, beta, L.Np, L.KTC,
-                               Bop);
+            hipLaunchKernelGGL((operand_tile_kernel<true, true>), grid, dim3(256), 0, st, B, n, k,
+                               ldb, cexp, slices, beta, L.Np, L.KTC, Bop);
         else
-            hipLaunchKernelGGL(operand_cols_kernel<false>, dim3((unsigned)L.KTC, (unsigned)gy),
-                               dim3(256), 0, st, B, k, n, ldb, cexp, slices, 0, L.Np, L.KTC, Bop);
-        if ((rc = mgx::launch_status("operand_cols_kernel")) != MGX_OK) return rc;
+            hipLaunchKernelGGL((operand_tile_kernel<false, true>), grid, dim3(256), 0, st, B, n, k,
+                               ldb, cexp, slices, 0, L.Np, L.KTC, Bop);
+        if ((rc = mgx::launch_status("operand_tile_kernel (B)")) != MGX_OK) return rc;
     }
     if ((rc = mark(2)) != MGX_OK) return rc;
     ProductArgs pa;
