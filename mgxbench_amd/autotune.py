@@ -7,8 +7,11 @@ per candidate: a clipped log-log power law or an lstsq fit on (x^2, x^3)
 fit ("ml", :140-165) or a hold-out vote between them ("auto", :213-233).
 
 Here the candidates are the knobs of the MI355X local product, which the
-reference does not have: form (FP64 MFMA or INT8 split with s slices), the
-kernel block tile and the SUMMA panel width.  One MLP (3 layers, width 64,
+reference does not have: form (FP64 MFMA, INT8 digit-slice split with s
+slices, or INT8 modular split with n moduli), the kernel block tile and the
+SUMMA panel width.  Only FP64-accurate candidates are offered: s >= 7 slices,
+and moduli counts whose scaling exponent at the panel depth is at least
+MIN_BETA (mgx_split_mod_beta).  One MLP (3 layers, width 64,
 FP32, trained on the GPU by libmgx's own kernels) predicts log runtime from
 log-shape features for every candidate at once; exhaustive timing of its
 top-k candidates stays the fallback and the judge.
@@ -40,7 +43,9 @@ from . import _lib
 
 SIZES = list(range(4096, 32768 + 1, 4096))
 PANELS = (128, 256, 512, 1024)
-SLICES = (6, 7, 8)
+SLICES = (7, 8)
+MODULI = (14, 15, 16)
+MIN_BETA = 49   # quantisation to >= 49 bits keeps the modular form within plain FP64's error
 TILES = (_lib.TILE_LARGE, _lib.TILE_SMALL)
 DEFAULT = None  # filled below: the reference's default configuration
 SCHEMA = 1
@@ -48,8 +53,8 @@ SCHEMA = 1
 
 @dataclass(frozen=True)
 class Candidate:
-    form: str        # "fp64" | "int8"
-    slices: int      # 0 for fp64
+    form: str        # "fp64" | "int8" | "mod"
+    slices: int      # slices (int8), moduli (mod), 0 for fp64
     tile: int        # _lib.TILE_LARGE | _lib.TILE_SMALL
     panel: int       # SUMMA panel width (BK)
 
@@ -58,6 +63,13 @@ class Candidate:
 
 
 DEFAULT = Candidate("fp64", 0, _lib.TILE_LARGE, 256)   # MGXBENCH_SUMMA_BK default 256
+
+
+FORM_CODE = {"fp64": _lib.FORM_FP64, "int8": _lib.FORM_INT8, "mod": _lib.FORM_INT8_MOD}
+
+
+def mod_beta(moduli: int, k: int) -> int:
+    return int(_lib.lib().mgx_split_mod_beta(moduli, k))
 
 
 def candidate_space(k: int) -> list[Candidate]:
@@ -69,6 +81,9 @@ def candidate_space(k: int) -> list[Candidate]:
             out.append(Candidate("fp64", 0, tile, panel))
             for s in SLICES:
                 out.append(Candidate("int8", s, tile, panel))
+            for nm in MODULI:
+                if mod_beta(nm, panel) >= MIN_BETA:
+                    out.append(Candidate("mod", nm, tile, panel))
     return out
 
 
@@ -93,18 +108,20 @@ def random_shapes(count: int, seed: int = 7, sizes=SIZES) -> list[tuple[int, int
 
 
 FEATURE_NAMES = ["log2_m", "log2_n", "log2_k", "log2_panel", "log2_panels", "ragged_frac",
-                 "int8", "slices", "pairs", "tile_small", "log2_mnk", "log2_mn"]
+                 "int8", "slices", "products", "tile_small", "log2_mnk", "log2_mn", "mod"]
 
 
 def features(m: int, n: int, k: int, c: Candidate) -> list[float]:
     panel = min(c.panel, k)
     panels = math.ceil(k / panel)
     ragged = (k - (panels - 1) * panel) / panel
-    pairs = c.slices * (c.slices + 1) / 2 if c.form == "int8" else 0.0
+    # INT8 products per output: slice pairs (int8) or moduli (mod)
+    products = (c.slices * (c.slices + 1) / 2 if c.form == "int8"
+                else float(c.slices) if c.form == "mod" else 0.0)
     return [math.log2(m), math.log2(n), math.log2(k), math.log2(panel), math.log2(panels), ragged,
-            1.0 if c.form == "int8" else 0.0, c.slices / 8.0, pairs / 36.0,
+            1.0 if c.form == "int8" else 0.0, c.slices / 8.0, products / 36.0,
             1.0 if c.tile == _lib.TILE_SMALL else 0.0, math.log2(m * n * k) / 3.0,
-            math.log2(m * n) / 2.0]
+            math.log2(m * n) / 2.0, 1.0 if c.form == "mod" else 0.0]
 
 
 # ---- measurement (GPU) --------------------------------------------------------------------
@@ -122,15 +139,15 @@ class Timer:
         self.C = torch.empty(max_m * max_n, dtype=torch.float64, device=dev)
         mgx.fill(self.A, max_m, max_k, max_k, 1, seed=seed)
         mgx.fill(self.B, max_k, max_n, max_n, 1, seed=seed + 1)
-        ws = max(mgx.split_workspace_bytes(mgx.FORM_INT8, max(SLICES), max_m, max_n, p)
+        ws = max(max(mgx.split_workspace_bytes(mgx.FORM_INT8, max(SLICES), max_m, max_n, p),
+                     mgx.split_workspace_bytes(mgx.FORM_INT8_MOD, max(MODULI), max_m, max_n, p))
                  for p in PANELS)
         self.ws = torch.empty(ws, dtype=torch.uint8, device=dev)
         self.ws_bytes = ws
         torch.cuda.synchronize()
 
     def time(self, m: int, n: int, k: int, c: Candidate, est_ms: float | None = None) -> float:
-        cfg = _lib.TuneConfig(_lib.FORM_FP64 if c.form == "fp64" else _lib.FORM_INT8, c.slices,
-                              c.tile, c.panel)
+        cfg = _lib.TuneConfig(FORM_CODE[c.form], c.slices, c.tile, c.panel)
         est = est_ms if est_ms is not None else 2.0 * m * n * k / 50e12 * 1e3
         warmup = 1 if est < 50 else 0
         reps = int(min(5, max(1, 100 // max(est, 1))))

@@ -54,13 +54,15 @@ struct Options {
     uint64_t seed = 1;
 };
 
-// MGX_FORM=fp64|int8, MGX_SLICES=<n>, MGX_INIT=const|uniform[:SEED]
+// MGX_FORM=fp64|int8|mod, MGX_SLICES=<n> (slices of int8, default 8; moduli
+// of mod, default 15), MGX_INIT=const|uniform[:SEED]
 inline bool parse_options(Options *o) {
     const std::string form = env_str("MGX_FORM", "fp64");
     if (form == "fp64") o->form = MGX_FORM_FP64;
     else if (form == "int8") o->form = MGX_FORM_INT8;
+    else if (form == "mod") o->form = MGX_FORM_INT8_MOD;
     else return false;
-    o->slices = env_int("MGX_SLICES", 8);
+    o->slices = env_int("MGX_SLICES", o->form == MGX_FORM_INT8_MOD ? 15 : 8);
     const std::string init = env_str("MGX_INIT", "const");
     if (init == "const") {
         o->uniform = false;
@@ -74,7 +76,9 @@ inline bool parse_options(Options *o) {
     return true;
 }
 
-inline const char *form_name(int form) { return form == MGX_FORM_INT8 ? "int8" : "fp64"; }
+inline const char *form_name(int form) {
+    return form == MGX_FORM_INT8 ? "int8" : form == MGX_FORM_INT8_MOD ? "mod" : "fp64";
+}
 
 // splitmix64 counter stream, the definition used by mgx_fill.
 inline uint64_t splitmix64_at(uint64_t seed, uint64_t i) {

@@ -18,7 +18,7 @@
 // The pipeline double-buffers chunk stages with "B ready" / "compute done"
 // events and runs C += A[:, k0:k0+kc] . B_chunk per chunk on a compute stream.
 //
-// Extensions (new env vars): MGX_FORM=fp64|int8, MGX_SLICES, MGX_INIT=
+// Extensions (new env vars): MGX_FORM=fp64|int8|mod, MGX_SLICES, MGX_INIT=
 // uniform:SEED (validated by the GPU double-double check), MGX_SHARE_RANKS=P
 // (rank 0's row block of a P-rank run: A_i is N/P x N).
 #include <chrono>
@@ -134,8 +134,8 @@ int main(int argc, char **argv) {
     }
     void *ws = nullptr;
     size_t ws_bytes = 0;
-    if (opt.form == MGX_FORM_INT8) {
-        ws_bytes = mgx_split_workspace_bytes(MGX_FORM_INT8, opt.slices, mi, n, pipelined ? kchunk : k);
+    if (opt.form != MGX_FORM_FP64) {
+        ws_bytes = mgx_split_workspace_bytes(opt.form, opt.slices, mi, n, pipelined ? kchunk : k);
         if (ws_bytes == 0) fail("MGX_SLICES out of range");
         DRV_HIP(hipMalloc(&ws, ws_bytes));
     }
@@ -145,7 +145,7 @@ int main(int argc, char **argv) {
         if (opt.form == MGX_FORM_FP64)
             DRV_MGX(mgx_dgemm_accumulate(d_A, Bc, d_C, mi, n, k, k_off, kc, k, n, n, TILE_DIM, st));
         else
-            DRV_MGX(mgx_split_gemm(MGX_FORM_INT8, opt.slices, d_A + k_off, Bc, d_C, mi, n, kc, k, n,
+            DRV_MGX(mgx_split_gemm(opt.form, opt.slices, d_A + k_off, Bc, d_C, mi, n, kc, k, n,
                                    n, 1, ws, ws_bytes, st));
     };
 
@@ -155,7 +155,7 @@ int main(int argc, char **argv) {
             if (opt.form == MGX_FORM_FP64)
                 DRV_MGX(mgx_dgemm(d_A, d_B, d_C, mi, n, k, k, n, n, TILE_DIM, nullptr));
             else
-                DRV_MGX(mgx_split_gemm(MGX_FORM_INT8, opt.slices, d_A, d_B, d_C, mi, n, k, k, n, n, 0,
+                DRV_MGX(mgx_split_gemm(opt.form, opt.slices, d_A, d_B, d_C, mi, n, k, k, n, n, 0,
                                        ws, ws_bytes, nullptr));
             DRV_HIP(hipGetLastError());
             DRV_HIP(hipDeviceSynchronize());

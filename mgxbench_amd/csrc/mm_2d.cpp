@@ -14,7 +14,7 @@
 // reference's P=1 paths do.
 //
 // Extensions (new env vars; the CLI and the result keys the runner checks are
-// unchanged): MGX_FORM=fp64|int8 and MGX_SLICES select the product form;
+// unchanged): MGX_FORM=fp64|int8|mod and MGX_SLICES select the product form;
 // MGX_INIT=uniform:SEED replaces the constant A=1, B=2 inputs (validated by
 // the GPU double-double check instead of the constant rule);
 // MGX_SHARE_GRID=RxC computes rank (0,0)'s share of an R x C grid at this N

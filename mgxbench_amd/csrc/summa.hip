@@ -81,7 +81,9 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
     MGX_ARG_CHECK(m > 0 && n > 0 && k > 0 && bk > 0, "sizes and panel width must be positive");
     MGX_ARG_CHECK(hA && hB && dC, "NULL matrix pointer");
     MGX_ARG_CHECK(lda >= k && ldb >= n && ldc >= n, "leading dimension too small");
-    MGX_ARG_CHECK(P->form == MGX_FORM_FP64 || P->form == MGX_FORM_INT8, "unsupported form");
+    MGX_ARG_CHECK(P->form == MGX_FORM_FP64 || P->form == MGX_FORM_INT8 ||
+                      P->form == MGX_FORM_INT8_MOD,
+                  "unsupported form");
     MGX_ARG_CHECK(P->copy_mode == MGX_SUMMA_COPY_ASYNC || P->copy_mode == MGX_SUMMA_COPY_HOST_SYNC,
                   "unknown copy mode");
     const int64_t panel = bk < k ? bk : k;
@@ -108,8 +110,8 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
     }
     MGX_HIP_TRY(hipEventCreateWithFlags(&R.fork, hipEventDisableTiming));
     MGX_HIP_TRY(hipEventCreateWithFlags(&R.join, hipEventDisableTiming));
-    if (P->form == MGX_FORM_INT8) {
-        R.ws_bytes = mgx_split_workspace_bytes(MGX_FORM_INT8, P->slices, m, n, panel);
+    if (P->form != MGX_FORM_FP64) {
+        R.ws_bytes = mgx_split_workspace_bytes(P->form, P->slices, m, n, panel);
         if (R.ws_bytes == 0) return mgx::set_error(MGX_ERR_UNSUPPORTED, "slice count out of range");
         if (hipMalloc(&R.ws, R.ws_bytes) != hipSuccess)
             return mgx::set_error(MGX_ERR_ALLOC, "split workspace allocation failed");
@@ -161,7 +163,7 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
                 rc = mgx_dgemm_accumulate(R.dA[cur], R.dB[cur], dC, m, n, kc, 0, kc, kc, n, ldc,
                                           32, R.compute);
             } else {
-                rc = mgx_split_gemm(MGX_FORM_INT8, P->slices, R.dA[cur], R.dB[cur], dC, m, n, kc,
+                rc = mgx_split_gemm(P->form, P->slices, R.dA[cur], R.dB[cur], dC, m, n, kc,
                                     kc, n, ldc, 1, R.ws, R.ws_bytes, R.compute);
             }
             if (rc != MGX_OK) return rc;

@@ -31,7 +31,9 @@ extern "C" int mgx_time_local_product(const mgx_tune_config *cfg, const double *
                                       double *ms_out) {
     mgx::clear_error();
     MGX_ARG_CHECK(cfg && ms_out, "NULL pointer");
-    MGX_ARG_CHECK(cfg->form == MGX_FORM_FP64 || cfg->form == MGX_FORM_INT8, "unsupported form");
+    MGX_ARG_CHECK(cfg->form == MGX_FORM_FP64 || cfg->form == MGX_FORM_INT8 ||
+                      cfg->form == MGX_FORM_INT8_MOD,
+                  "unsupported form");
     MGX_ARG_CHECK(cfg->panel > 0 && m > 0 && n > 0 && k > 0, "invalid sizes");
     MGX_ARG_CHECK(warmup >= 0 && reps > 0, "invalid repetition counts");
     hipStream_t st = nullptr;

@@ -12,8 +12,15 @@ from mgxbench_amd import _lib
 
 def test_candidate_space():
     c = at.candidate_space(16384)
-    assert len(c) == len(at.PANELS) * len(at.TILES) * (1 + len(at.SLICES))
+    mods = [x for x in c if x.form == "mod"]
+    assert len(c) == len(at.PANELS) * len(at.TILES) * (1 + len(at.SLICES)) + len(mods)
     assert at.DEFAULT in c
+    # only FP64-accurate candidates: s >= 7, and moduli with beta >= MIN_BETA at the panel
+    assert all(x.slices >= 7 for x in c if x.form == "int8")
+    assert mods and all(at.mod_beta(x.slices, x.panel) >= at.MIN_BETA for x in mods)
+    # panel depths <= 1024 keep beta >= 49 down to 14 moduli (beta(14, 1024) = 49)
+    assert {x.slices for x in mods if x.panel == 1024} == {14, 15, 16}
+    assert at.mod_beta(14, 1024) == 49 and at.mod_beta(13, 128) == 47
     small = at.candidate_space(200)     # panels wider than K are dropped
     assert {x.panel for x in small} == {128}
     assert len({x.key() for x in c}) == len(c)

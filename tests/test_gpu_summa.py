@@ -129,6 +129,22 @@ def test_int8_form_panels_bit_exact(mgx, orc):
     assert np.array_equal(C, ref)
 
 
+def test_modular_form_panels_bit_exact(mgx, orc):
+    # each panel product is the oracle's modular model at that panel depth,
+    # accumulated into C in panel order
+    m, n, k, panel = 190, 170, 600, 256
+    announce("summa modular form")
+    A = orc.fill(m, k, 1, seed=43)
+    B = orc.fill(k, n, 1, seed=44)
+    C, _ = _run(mgx, A, B, panel, pinned=True, form=mgx.FORM_INT8_MOD, slices=15)
+    ref = np.zeros((m, n))
+    for k0 in range(0, k, panel):
+        kc = min(panel, k - k0)
+        ref = orc.crt_gemm(np.ascontiguousarray(A[:, k0:k0 + kc]),
+                           np.ascontiguousarray(B[k0:k0 + kc]), 15, C=ref)
+    assert np.array_equal(C, ref)
+
+
 def test_invalid_summa_arguments(mgx):
     A = np.zeros((4, 4))
     dC = empty(4, 4)
