@@ -714,9 +714,10 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINB) void int8_product_kernel(Prod
     }
 }
 
-// Scheme II reconstruction: one thread per (block tile, wave, lane) of the
-// product kernel's layout; for each of its outputs it gathers the n residues,
-// recovers C' exactly, rounds once and scales.
+// Scheme II reconstruction: one thread per (block tile, wave, fragment
+// quad, lane) of the product kernel's layout, so each wave reads one
+// contiguous 1 KiB run per modulus; for each of its 16 outputs it gathers the
+// n residues, recovers C' exactly, rounds once and scales.
 template <int TB, int WGM, int WGN>
 __global__ __launch_bounds__(256) void reconstruct_kernel(const uint8_t *__restrict__ R,
                                                           const int32_t *__restrict__ rexp,
@@ -729,17 +730,17 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(const uint8_t *__restr
     const int tiles_n = (int)((N + TB - 1) / TB);
     const int64_t tiles = (int64_t)tiles_m * tiles_n;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= tiles * S::THREADS) return;
-    const int64_t tile = gid / S::THREADS;
-    const int wave = (int)((gid / 64) % S::NW);
+    if (gid >= tiles * S::THREADS * S::QPL) return;
     const int lane = (int)(gid % 64);
+    const int q = (int)((gid / 64) % S::QPL);
+    const int wave = (int)((gid / (64 * S::QPL)) % S::NW);
+    const int64_t tile = gid / ((int64_t)S::THREADS * S::QPL);
     const int64_t m0 = (tile / tiles_n) * TB;
     const int64_t n0 = (tile % tiles_n) * TB;
     const int64_t row0 = m0 + (wave / WGN) * S::WTM + 4 * (lane >> 4);
     const int64_t col0 = n0 + (wave % WGN) * S::WTN + (lane & 15);
     const int shift = -2 * beta;
-#pragma unroll 1
-    for (int q = 0; q < S::QPL; ++q) {
+    {
         v4i runs[MAX_MODULI];
 #pragma unroll
         for (int l = 0; l < MAX_MODULI; ++l)
@@ -992,7 +993,8 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     if ((rc = mgx::launch_status("int8_product_kernel")) != MGX_OK) return rc;
     if ((rc = mark(3)) != MGX_OK) return rc;
     if (mod) {
-        const int threads = tile == MGX_TILE_LARGE ? ShapeLarge::THREADS : ShapeSmall::THREADS;
+        const int threads = tile == MGX_TILE_LARGE ? ShapeLarge::THREADS * ShapeLarge::QPL
+                                                   : ShapeSmall::THREADS * ShapeSmall::QPL;
         const int64_t total = tiles * threads;
         const int64_t blocks = (total + 255) / 256;
         MGX_ARG_CHECK(blocks <= 0x7fffffffll, "matrix too large for the reconstruction");
