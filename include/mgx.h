@@ -198,7 +198,12 @@ int mgx_error_report(const double *A, const double *B, const double *C,
  * strided DMA; pageable strips are packed into library-owned pinned
  * buffers first (the reference's pack_a / pack_b).  dC is a device m x n
  * matrix that is accumulated into (zero it for C = A.B).  Synchronous: all
- * resources are created before and released after the timed loop. */
+ * resources are created before and released after the timed loop.
+ * Split forms: panels still stream one at a time, but consecutive panels
+ * are gathered into a group of mgx_split_group_k(...) columns and
+ * multiplied by one split product (C += A_g . B_g per group), so the
+ * exponent scan, operand conversion and reconstruction are paid per group,
+ * not per panel.  The FP64 form multiplies each panel as it lands. */
 enum {
     MGX_SUMMA_COPY_ASYNC = 0,      /* copies overlap compute (MMM, YYY, NNN codes) */
     MGX_SUMMA_COPY_HOST_SYNC = 1   /* host waits for each panel copy (CCC, XXX, WWW, SSS) */
@@ -207,11 +212,13 @@ enum {
 typedef struct mgx_summa_params {
     int64_t m, n, k;    /* local C is m x n; reduction length k */
     int64_t panel;      /* panel width (the reference's MGXBENCH_SUMMA_BK) */
-    int form;           /* MGX_FORM_FP64 or MGX_FORM_INT8 */
-    int slices;         /* slices of the INT8 form */
+    int form;           /* MGX_FORM_FP64, MGX_FORM_INT8 or MGX_FORM_INT8_MOD */
+    int slices;         /* slices (INT8) or moduli (INT8_MOD) */
     int copy_mode;      /* MGX_SUMMA_COPY_* */
     int use_graph;      /* capture the whole panel loop in one hipGraph (pinned strips) */
     int serialize;      /* debug: synchronise after every copy and every product */
+    int64_t split_k;    /* split forms: columns per split product (whole panels);
+                           <= 0 selects MGX_SPLIT_GROUP_K */
 } mgx_summa_params;
 
 typedef struct mgx_summa_stats {
@@ -219,6 +226,16 @@ typedef struct mgx_summa_stats {
     int64_t panels;     /* number of panels, including a ragged last one */
     int host_pinned;    /* 1 if the host strips were pinned (no host packing) */
 } mgx_summa_stats;
+
+/* Default reduction columns per split product in a panel loop. */
+#define MGX_SPLIT_GROUP_K 4096
+
+/* Reduction columns of each split product in a panel loop (SUMMA and the
+ * autotuner's local product): the smallest whole number of panels covering
+ * split_k columns (MGX_SPLIT_GROUP_K when split_k <= 0), capped at k; for
+ * MGX_FORM_FP64 one panel.  The last group may be shorter.  Returns -1 for
+ * non-positive panel or k, or an unknown form. */
+int64_t mgx_split_group_k(int form, int64_t panel, int64_t split_k, int64_t k);
 
 int mgx_summa_run(const mgx_summa_params *params,
                   const double *hA, int64_t lda,
@@ -233,16 +250,19 @@ int mgx_host_free(void *ptr);
 /* ---- autotuner measurement ------------------------------------------------
  * One candidate configuration of the local product: form (MGX_FORM_FP64 or
  * MGX_FORM_INT8), slices (INT8), tile (MGX_TILE_*) and SUMMA panel width.
- * mgx_time_local_product runs C = sum over panels of A[:, k0:k0+kc] .
- * B[k0:k0+kc, :] with device-resident operands (first panel overwrites C,
- * later panels accumulate), `warmup` untimed then `reps` timed repetitions,
- * and returns the mean milliseconds per full product (hipEvent timing).
- * workspace may be NULL (library scratch) for the INT8 form. */
+ * mgx_time_local_product runs the compute side of the SUMMA panel loop on
+ * device-resident operands: C = sum over groups g of A[:, g] . B[g, :], with
+ * groups of one panel (FP64) or mgx_split_group_k(form, panel, split_k, k)
+ * columns (split forms); the first group overwrites C, later groups
+ * accumulate.  `warmup` untimed then `reps` timed repetitions; returns the
+ * mean milliseconds per full product (hipEvent timing).  workspace may be
+ * NULL (library scratch). */
 typedef struct mgx_tune_config {
     int form;
     int slices;
     int tile;
     int64_t panel;
+    int64_t split_k;    /* as mgx_summa_params.split_k */
 } mgx_tune_config;
 
 int mgx_time_local_product(const mgx_tune_config *cfg,

@@ -41,7 +41,7 @@ class SummaParams(ctypes.Structure):
     _fields_ = [("m", ctypes.c_int64), ("n", ctypes.c_int64), ("k", ctypes.c_int64),
                 ("panel", ctypes.c_int64), ("form", ctypes.c_int), ("slices", ctypes.c_int),
                 ("copy_mode", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("serialize", ctypes.c_int)]
+                ("serialize", ctypes.c_int), ("split_k", ctypes.c_int64)]
 
 
 class SummaStats(ctypes.Structure):
@@ -51,7 +51,7 @@ class SummaStats(ctypes.Structure):
 
 class TuneConfig(ctypes.Structure):
     _fields_ = [("form", ctypes.c_int), ("slices", ctypes.c_int), ("tile", ctypes.c_int),
-                ("panel", ctypes.c_int64)]
+                ("panel", ctypes.c_int64), ("split_k", ctypes.c_int64)]
 
 
 TILE_DEFAULT = 0
@@ -84,6 +84,7 @@ _SIGNATURES = {
     "mgx_split_mod_beta": (_int, [_int, _i64]),
     "mgx_error_report": (_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _i64,
                                 ctypes.POINTER(ErrorStats)]),
+    "mgx_split_group_k": (_i64, [_int, _i64, _i64, _i64]),
     "mgx_summa_run": (_int, [ctypes.POINTER(SummaParams), _vp, _i64, _vp, _i64, _vp, _i64,
                              ctypes.POINTER(SummaStats)]),
     "mgx_host_alloc": (_int, [ctypes.POINTER(_vp), ctypes.c_size_t]),

@@ -72,6 +72,12 @@ def mod_beta(moduli: int, k: int) -> int:
     return int(_lib.lib().mgx_split_mod_beta(moduli, k))
 
 
+def step_k(c: Candidate, k: int) -> int:
+    """Reduction depth of each product of the panel loop: one panel (FP64) or
+    a group of whole panels (split forms; mgx_split_group_k, default group)."""
+    return int(_lib.lib().mgx_split_group_k(FORM_CODE[c.form], c.panel, 0, k))
+
+
 def candidate_space(k: int) -> list[Candidate]:
     out = []
     for panel in PANELS:
@@ -82,8 +88,9 @@ def candidate_space(k: int) -> list[Candidate]:
             for s in SLICES:
                 out.append(Candidate("int8", s, tile, panel))
             for nm in MODULI:
-                if mod_beta(nm, panel) >= MIN_BETA:
-                    out.append(Candidate("mod", nm, tile, panel))
+                c = Candidate("mod", nm, tile, panel)
+                if mod_beta(nm, step_k(c, k)) >= MIN_BETA:
+                    out.append(c)
     return out
 
 
@@ -108,20 +115,25 @@ def random_shapes(count: int, seed: int = 7, sizes=SIZES) -> list[tuple[int, int
 
 
 FEATURE_NAMES = ["log2_m", "log2_n", "log2_k", "log2_panel", "log2_panels", "ragged_frac",
-                 "int8", "slices", "products", "tile_small", "log2_mnk", "log2_mn", "mod"]
+                 "int8", "slices", "products", "tile_small", "log2_mnk", "log2_mn", "mod",
+                 "log2_step_k", "log2_steps"]
 
 
 def features(m: int, n: int, k: int, c: Candidate) -> list[float]:
     panel = min(c.panel, k)
     panels = math.ceil(k / panel)
     ragged = (k - (panels - 1) * panel) / panel
+    # products of the panel loop: one per panel (FP64) or per panel group
+    sk = step_k(c, k)
+    steps = math.ceil(k / sk)
     # INT8 products per output: slice pairs (int8) or moduli (mod)
     products = (c.slices * (c.slices + 1) / 2 if c.form == "int8"
                 else float(c.slices) if c.form == "mod" else 0.0)
     return [math.log2(m), math.log2(n), math.log2(k), math.log2(panel), math.log2(panels), ragged,
             1.0 if c.form == "int8" else 0.0, c.slices / 8.0, products / 36.0,
             1.0 if c.tile == _lib.TILE_SMALL else 0.0, math.log2(m * n * k) / 3.0,
-            math.log2(m * n) / 2.0, 1.0 if c.form == "mod" else 0.0]
+            math.log2(m * n) / 2.0, 1.0 if c.form == "mod" else 0.0,
+            math.log2(sk), math.log2(steps)]
 
 
 # ---- measurement (GPU) --------------------------------------------------------------------
@@ -139,9 +151,9 @@ class Timer:
         self.C = torch.empty(max_m * max_n, dtype=torch.float64, device=dev)
         mgx.fill(self.A, max_m, max_k, max_k, 1, seed=seed)
         mgx.fill(self.B, max_k, max_n, max_n, 1, seed=seed + 1)
-        ws = max(max(mgx.split_workspace_bytes(mgx.FORM_INT8, max(SLICES), max_m, max_n, p),
-                     mgx.split_workspace_bytes(mgx.FORM_INT8_MOD, max(MODULI), max_m, max_n, p))
-                 for p in PANELS)
+        ws = max(max(mgx.split_workspace_bytes(mgx.FORM_INT8, max(SLICES), max_m, max_n, g),
+                     mgx.split_workspace_bytes(mgx.FORM_INT8_MOD, max(MODULI), max_m, max_n, g))
+                 for g in {step_k(Candidate("mod", 0, _lib.TILE_LARGE, p), max_k) for p in PANELS})
         self.ws = torch.empty(ws, dtype=torch.uint8, device=dev)
         self.ws_bytes = ws
         torch.cuda.synchronize()

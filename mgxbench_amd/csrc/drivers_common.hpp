@@ -55,7 +55,9 @@ struct Options {
 };
 
 // MGX_FORM=fp64|int8|mod, MGX_SLICES=<n> (slices of int8, default 8; moduli
-// of mod, default 15), MGX_INIT=const|uniform[:SEED]
+// of mod, default 15), MGX_INIT=const|uniform[:SEED].  Split forms also read
+// MGX_SPLIT_K (columns per split product in the panel / chunk loops, whole
+// panels or chunks; default MGX_SPLIT_GROUP_K) where the loop runs.
 inline bool parse_options(Options *o) {
     const std::string form = env_str("MGX_FORM", "fp64");
     if (form == "fp64") o->form = MGX_FORM_FP64;

@@ -114,7 +114,12 @@ int main(int argc, char **argv) {
     if (warmup_count < 0) fail("MGXBENCH_INTERNAL_WARMUP must be non-negative", 4);
 
     const bool pipelined = libB == 'S' || libB == 'Y' || libB == 'X';
-    const int kchunk = pipeline_k_chunk((int)k);
+    // split forms stage whole groups of chunks per split product
+    // (MGX_SPLIT_K columns, default MGX_SPLIT_GROUP_K)
+    const int kchunk = opt.form == MGX_FORM_FP64
+                           ? pipeline_k_chunk((int)k)
+                           : (int)mgx_split_group_k(opt.form, pipeline_k_chunk((int)k),
+                                                    env_int("MGX_SPLIT_K", 0), k);
     hipStream_t comm = nullptr, compute = nullptr;
     hipEvent_t b_ready[2] = {}, compute_done[2] = {};
     double *b_stage[2] = {nullptr, nullptr};

@@ -124,6 +124,7 @@ int main(int argc, char **argv) {
     P.copy_mode = async_copy ? MGX_SUMMA_COPY_ASYNC : MGX_SUMMA_COPY_HOST_SYNC;
     P.use_graph = graph ? 1 : 0;
     P.serialize = env_int("MGX_SUMMA_SERIALIZE", 0) != 0;
+    P.split_k = env_int("MGX_SPLIT_K", 0);
     mgx_summa_stats S;
     if (mgx_summa_run(&P, hA, N, hB, n, dC, n, &S) != MGX_OK) fail(mgx_last_error());
     const double elapsed = S.seconds;

@@ -9,6 +9,12 @@
 // per-buffer "panel ready" event and runs the accumulate product; a
 // "compute done" event guards reuse of each of the two device buffers.
 //
+// Split forms gather consecutive panels into a group buffer (whole panels
+// covering mgx_split_group_k columns) and run one split product per group:
+// the panel copies keep the reference's granularity and overlap the previous
+// group's product, while the exponent scan, operand conversion and (modular
+// form) reconstruction of the whole C are paid once per group.
+//
 // MI355X-native differences from the reference's loop:
 //  * With pinned host strips the A panel is gathered by the DMA engine
 //    (strided 2D copy) instead of host memcpy packing; with pageable strips
@@ -32,6 +38,7 @@ namespace {
 struct SummaResources {
     hipStream_t compute = nullptr, transfer = nullptr;
     hipEvent_t ready[2] = {nullptr, nullptr}, done[2] = {nullptr, nullptr};
+    hipEvent_t staged[2] = {nullptr, nullptr};
     hipEvent_t fork = nullptr, join = nullptr;
     double *dA[2] = {nullptr, nullptr}, *dB[2] = {nullptr, nullptr};
     double *hA[2] = {nullptr, nullptr}, *hB[2] = {nullptr, nullptr};
@@ -48,6 +55,7 @@ struct SummaResources {
         for (int i = 0; i < 2; ++i) {
             if (ready[i]) (void)hipEventDestroy(ready[i]);
             if (done[i]) (void)hipEventDestroy(done[i]);
+            if (staged[i]) (void)hipEventDestroy(staged[i]);
             if (dA[i]) (void)hipFree(dA[i]);
             if (dB[i]) (void)hipFree(dB[i]);
             if (hA[i]) (void)hipHostFree(hA[i]);
@@ -72,6 +80,16 @@ bool host_is_pinned(const void *p) {
 
 }  // namespace
 
+extern "C" int64_t mgx_split_group_k(int form, int64_t panel, int64_t split_k, int64_t k) {
+    if (panel <= 0 || k <= 0) return -1;
+    if (form != MGX_FORM_FP64 && form != MGX_FORM_INT8 && form != MGX_FORM_INT8_MOD) return -1;
+    const int64_t p = panel < k ? panel : k;
+    if (form == MGX_FORM_FP64) return p;
+    const int64_t want = split_k > 0 ? split_k : MGX_SPLIT_GROUP_K;
+    const int64_t panels = (want + p - 1) / p;
+    return panels * p < k ? panels * p : k;
+}
+
 extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_t lda,
                              const double *hB, int64_t ldb, double *dC, int64_t ldc,
                              mgx_summa_stats *stats) {
@@ -88,6 +106,9 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
                   "unknown copy mode");
     const int64_t panel = bk < k ? bk : k;
     const int64_t npanels = (k + panel - 1) / panel;
+    // group width: one panel (FP64) or whole panels per split product
+    const int64_t gw = mgx_split_group_k(P->form, panel, P->split_k, k);
+    const int64_t ngroups = (k + gw - 1) / gw;
     const bool pinned = host_is_pinned(hA) && host_is_pinned(hB);
     const bool graph = P->use_graph != 0;
     MGX_ARG_CHECK(!graph || (pinned && P->copy_mode == MGX_SUMMA_COPY_ASYNC && !P->serialize),
@@ -99,8 +120,9 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
     for (int i = 0; i < 2; ++i) {
         MGX_HIP_TRY(hipEventCreateWithFlags(&R.ready[i], hipEventDisableTiming));
         MGX_HIP_TRY(hipEventCreateWithFlags(&R.done[i], hipEventDisableTiming));
-        if (hipMalloc(&R.dA[i], sizeof(double) * m * panel) != hipSuccess ||
-            hipMalloc(&R.dB[i], sizeof(double) * panel * n) != hipSuccess)
+        MGX_HIP_TRY(hipEventCreateWithFlags(&R.staged[i], hipEventDisableTiming));
+        if (hipMalloc(&R.dA[i], sizeof(double) * m * gw) != hipSuccess ||
+            hipMalloc(&R.dB[i], sizeof(double) * gw * n) != hipSuccess)
             return mgx::set_error(MGX_ERR_ALLOC, "panel buffer allocation failed");
         if (!pinned) {
             if (hipHostMalloc((void **)&R.hA[i], sizeof(double) * m * panel) != hipSuccess ||
@@ -111,60 +133,65 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
     MGX_HIP_TRY(hipEventCreateWithFlags(&R.fork, hipEventDisableTiming));
     MGX_HIP_TRY(hipEventCreateWithFlags(&R.join, hipEventDisableTiming));
     if (P->form != MGX_FORM_FP64) {
-        R.ws_bytes = mgx_split_workspace_bytes(P->form, P->slices, m, n, panel);
+        R.ws_bytes = mgx_split_workspace_bytes(P->form, P->slices, m, n, gw);
         if (R.ws_bytes == 0) return mgx::set_error(MGX_ERR_UNSUPPORTED, "slice count out of range");
         if (hipMalloc(&R.ws, R.ws_bytes) != hipSuccess)
             return mgx::set_error(MGX_ERR_ALLOC, "split workspace allocation failed");
     }
 
-    // Enqueues the panel loop on the two streams.  When `host_sync` is set
-    // (pageable strips, or a copy mode that waits on the host), the host
-    // waits for buffer reuse and packs panels itself.
-    auto enqueue = [&](bool capturing) -> int {
+    // Enqueues the panel loop on the two streams.  Device group buffers are
+    // reused under stream waits on "compute done"; with pageable strips the
+    // host packs each panel into one of two pinned staging buffers after the
+    // copy that last read it has finished.
+    auto enqueue = [&]() -> int {
         int rc;
         MGX_HIP_TRY(hipEventRecord(R.fork, R.compute));
         MGX_HIP_TRY(hipStreamWaitEvent(R.transfer, R.fork, 0));
-        for (int64_t it = 0; it < npanels; ++it) {
-            const int cur = (int)(it & 1);
-            const int64_t k0 = it * panel;
-            const int64_t kc = (k - k0) < panel ? (k - k0) : panel;
-            if (it >= 2) {
-                if (!pinned && !capturing) {
-                    MGX_HIP_TRY(hipEventSynchronize(R.done[cur]));
-                } else {
-                    MGX_HIP_TRY(hipStreamWaitEvent(R.transfer, R.done[cur], 0));
+        int64_t pi = 0;   // panel counter (staging buffer parity)
+        for (int64_t g = 0; g < ngroups; ++g) {
+            const int cur = (int)(g & 1);
+            const int64_t g0 = g * gw;
+            const int64_t gk = (k - g0) < gw ? (k - g0) : gw;
+            if (g >= 2) MGX_HIP_TRY(hipStreamWaitEvent(R.transfer, R.done[cur], 0));
+            for (int64_t j0 = 0; j0 < gk; j0 += panel, ++pi) {
+                const int64_t kc = (gk - j0) < panel ? (gk - j0) : panel;
+                const int64_t k0 = g0 + j0;
+                const double *srcA = hA + k0;
+                const double *srcB = hB + k0 * ldb;
+                int64_t spitchA = lda, spitchB = ldb;
+                const int s = (int)(pi & 1);
+                if (!pinned) {
+                    // pack_a / pack_b into the pinned staging buffer
+                    if (pi >= 2) MGX_HIP_TRY(hipEventSynchronize(R.staged[s]));
+                    for (int64_t i = 0; i < m; ++i)
+                        std::memcpy(R.hA[s] + i * kc, hA + i * lda + k0, sizeof(double) * kc);
+                    for (int64_t r = 0; r < kc; ++r)
+                        std::memcpy(R.hB[s] + r * n, hB + (k0 + r) * ldb, sizeof(double) * n);
+                    srcA = R.hA[s];
+                    srcB = R.hB[s];
+                    spitchA = kc;
+                    spitchB = n;
                 }
+                // panel j0 of the group: columns j0.. of the m x gk A buffer
+                // (lda = gk) and rows j0.. of the gk x n B buffer
+                MGX_HIP_TRY(hipMemcpy2DAsync(R.dA[cur] + j0, sizeof(double) * gk, srcA,
+                                             sizeof(double) * spitchA, sizeof(double) * kc, m,
+                                             hipMemcpyHostToDevice, R.transfer));
+                MGX_HIP_TRY(hipMemcpy2DAsync(R.dB[cur] + j0 * n, sizeof(double) * n, srcB,
+                                             sizeof(double) * spitchB, sizeof(double) * n, kc,
+                                             hipMemcpyHostToDevice, R.transfer));
+                if (!pinned) MGX_HIP_TRY(hipEventRecord(R.staged[s], R.transfer));
+                if (P->copy_mode == MGX_SUMMA_COPY_HOST_SYNC || P->serialize)
+                    MGX_HIP_TRY(hipStreamSynchronize(R.transfer));
             }
-            const double *srcA = hA + k0;
-            const double *srcB = hB + k0 * ldb;
-            int64_t spitchA = lda, spitchB = ldb;
-            if (!pinned) {
-                // pack_a / pack_b into the pinned staging buffer
-                for (int64_t i = 0; i < m; ++i)
-                    std::memcpy(R.hA[cur] + i * kc, hA + i * lda + k0, sizeof(double) * kc);
-                for (int64_t r = 0; r < kc; ++r)
-                    std::memcpy(R.hB[cur] + r * n, hB + (k0 + r) * ldb, sizeof(double) * n);
-                srcA = R.hA[cur];
-                srcB = R.hB[cur];
-                spitchA = kc;
-                spitchB = n;
-            }
-            MGX_HIP_TRY(hipMemcpy2DAsync(R.dA[cur], sizeof(double) * kc, srcA,
-                                         sizeof(double) * spitchA, sizeof(double) * kc, m,
-                                         hipMemcpyHostToDevice, R.transfer));
-            MGX_HIP_TRY(hipMemcpy2DAsync(R.dB[cur], sizeof(double) * n, srcB,
-                                         sizeof(double) * spitchB, sizeof(double) * n, kc,
-                                         hipMemcpyHostToDevice, R.transfer));
-            if (P->copy_mode == MGX_SUMMA_COPY_HOST_SYNC || P->serialize)
-                MGX_HIP_TRY(hipStreamSynchronize(R.transfer));
             MGX_HIP_TRY(hipEventRecord(R.ready[cur], R.transfer));
             MGX_HIP_TRY(hipStreamWaitEvent(R.compute, R.ready[cur], 0));
             if (P->form == MGX_FORM_FP64) {
-                rc = mgx_dgemm_accumulate(R.dA[cur], R.dB[cur], dC, m, n, kc, 0, kc, kc, n, ldc,
+                rc = mgx_dgemm_accumulate(R.dA[cur], R.dB[cur], dC, m, n, gk, 0, gk, gk, n, ldc,
                                           32, R.compute);
             } else {
-                rc = mgx_split_gemm(P->form, P->slices, R.dA[cur], R.dB[cur], dC, m, n, kc,
-                                    kc, n, ldc, 1, R.ws, R.ws_bytes, R.compute);
+                rc = mgx_split_gemm(P->form, P->slices, R.dA[cur], R.dB[cur], dC, m, n, gk,
+                                    gk, n, ldc, 1, R.ws, R.ws_bytes, R.compute);
             }
             if (rc != MGX_OK) return rc;
             MGX_HIP_TRY(hipEventRecord(R.done[cur], R.compute));
@@ -178,7 +205,7 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
     int rc;
     if (graph) {
         MGX_HIP_TRY(hipStreamBeginCapture(R.compute, hipStreamCaptureModeThreadLocal));
-        rc = enqueue(true);
+        rc = enqueue();
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(R.compute, &g);
         R.graph = g;
@@ -194,7 +221,7 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
     if (graph) {
         MGX_HIP_TRY(hipGraphLaunch(R.exec, R.compute));
     } else {
-        if ((rc = enqueue(false)) != MGX_OK) return rc;
+        if ((rc = enqueue()) != MGX_OK) return rc;
     }
     MGX_HIP_TRY(hipStreamSynchronize(R.compute));
     const auto t1 = std::chrono::steady_clock::now();

@@ -1,5 +1,6 @@
 // Autotuner measurement: time one candidate configuration of the local
-// product (form, slices, tile, panel width) on device-resident operands.
+// product (form, slices, tile, panel width, split group) on device-resident
+// operands.
 #include "mgx_internal.hpp"
 
 namespace {
@@ -7,7 +8,9 @@ namespace {
 int run_once(const mgx_tune_config *c, const double *A, const double *B, double *C, int64_t m,
              int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc, void *ws,
              size_t ws_bytes, hipStream_t st) {
-    const int64_t panel = c->panel < k ? c->panel : k;
+    // one panel per FP64 product, whole-panel groups per split product (the
+    // SUMMA loop's compute side)
+    const int64_t panel = mgx_split_group_k(c->form, c->panel, c->split_k, k);
     for (int64_t k0 = 0; k0 < k; k0 += panel) {
         const int64_t kc = (k - k0) < panel ? (k - k0) : panel;
         const int acc = k0 > 0 ? 1 : 0;

@@ -15,15 +15,31 @@ def test_candidate_space():
     mods = [x for x in c if x.form == "mod"]
     assert len(c) == len(at.PANELS) * len(at.TILES) * (1 + len(at.SLICES)) + len(mods)
     assert at.DEFAULT in c
-    # only FP64-accurate candidates: s >= 7, and moduli with beta >= MIN_BETA at the panel
+    # only FP64-accurate candidates: s >= 7, and moduli with beta >= MIN_BETA at
+    # the depth of each split product (a group of whole panels)
     assert all(x.slices >= 7 for x in c if x.form == "int8")
-    assert mods and all(at.mod_beta(x.slices, x.panel) >= at.MIN_BETA for x in mods)
-    # panel depths <= 1024 keep beta >= 49 down to 14 moduli (beta(14, 1024) = 49)
-    assert {x.slices for x in mods if x.panel == 1024} == {14, 15, 16}
-    assert at.mod_beta(14, 1024) == 49 and at.mod_beta(13, 128) == 47
+    assert mods and all(at.mod_beta(x.slices, at.step_k(x, 16384)) >= at.MIN_BETA for x in mods)
+    # groups of 4096 need 15 moduli (beta(14, 4096) = 48); at k = 1024 the
+    # whole depth is one group and 14 moduli keep beta = 49
+    assert {x.slices for x in mods} == {15, 16}
+    assert at.mod_beta(14, 4096) == 48 and at.mod_beta(14, 1024) == 49
+    assert {x.slices for x in at.candidate_space(1024) if x.form == "mod"} == {14, 15, 16}
     small = at.candidate_space(200)     # panels wider than K are dropped
     assert {x.panel for x in small} == {128}
     assert len({x.key() for x in c}) == len(c)
+
+
+def test_step_depth_is_one_panel_or_a_whole_panel_group():
+    L = _lib.TILE_LARGE
+    assert at.step_k(at.Candidate("fp64", 0, L, 256), 16384) == 256
+    assert at.step_k(at.Candidate("fp64", 0, L, 256), 100) == 100
+    assert at.step_k(at.Candidate("mod", 15, L, 256), 16384) == 4096
+    assert at.step_k(at.Candidate("int8", 7, L, 1000), 16384) == 5000    # whole panels
+    assert at.step_k(at.Candidate("mod", 15, L, 1024), 3000) == 3000     # capped at k
+    lib = _lib.lib()
+    assert lib.mgx_split_group_k(_lib.FORM_INT8_MOD, 128, 300, 16384) == 384
+    assert lib.mgx_split_group_k(_lib.FORM_INT8_MOD, 0, 0, 16384) == -1
+    assert lib.mgx_split_group_k(99, 128, 0, 16384) == -1
 
 
 def test_rank_shares_are_the_grid_shares():

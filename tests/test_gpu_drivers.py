@@ -46,7 +46,9 @@ def test_mm_1d_int8_and_uniform():
                 {"MGX_FORM": "int8", "MGX_INIT": "uniform:5", "MGX_SHARE_RANKS": "4"},
                 {"MGX_FORM": "mod", "MGX_INIT": "uniform:5"},
                 {"MGX_FORM": "mod", "MGX_SLICES": "16", "MGX_INIT": "uniform:5",
-                 "MGX_SHARE_RANKS": "4"}):
+                 "MGX_SHARE_RANKS": "4"},
+                {"MGX_FORM": "mod", "MGX_INIT": "uniform:5", "MM_PIPELINE_K_CHUNK": "256",
+                 "MGX_SPLIT_K": "300"}):
         proc, out = _run("mm_1d", ["0", "1024", "SSS"], {"MGXBENCH_INNER_LOOPS": "1", **env})
         assert proc.returncode == 0, out
         assert parse_result(out)["valid"] == "1"
@@ -73,7 +75,9 @@ def test_mm_2d_variants():
                 {"MGX_INIT": "uniform:2", "MGX_SHARE_GRID": "2x2"},
                 {"MGX_INIT": "uniform:2", "MGX_FORM": "int8", "MGX_SUMMA_GRAPH": "1"},
                 {"MGX_INIT": "uniform:2", "MGX_FORM": "mod", "MGX_SUMMA_GRAPH": "1"},
-                {"MGX_FORM": "mod", "MGX_SHARE_GRID": "2x2"}):
+                {"MGX_FORM": "mod", "MGX_SHARE_GRID": "2x2"},
+                {"MGX_INIT": "uniform:2", "MGX_FORM": "mod", "MGX_SUMMA_BK": "128",
+                 "MGX_SPLIT_K": "300"}):
         proc, out = _run("mm_2d", ["0", "1024", "NNN"], env)
         assert proc.returncode == 0, out
         assert parse_result(out)["valid"] == "1"

@@ -5,8 +5,10 @@ The panel loop must equal the reference's: C += A[:, k0:k0+kc] . B[k0:k0+kc, :]
 per panel in order (2D-block-matrix/mmb.cu:239-361), ragged last panel
 included.  Integer inputs make every order exact, so those cases compare bit
 for bit; real inputs are compared with the per-panel reference-order oracle
-within 2 gamma_kc |A||B| per panel.  Overlapped and serialized runs, graph and
-stream runs, pinned and pageable strips must agree bit for bit.
+within 2 gamma_kc |A||B| per panel.  Split forms multiply groups of whole
+panels, each group bit-exact against the oracle's model at the group's depth.
+Overlapped and serialized runs, graph and stream runs, pinned and pageable
+strips must agree bit for bit.
 """
 import ctypes
 
@@ -34,7 +36,7 @@ class PinnedArray:
 
 
 def _run(mgx, A, B, panel, *, pinned, form=0, slices=8, graph=False, serialize=False,
-         copy_mode=0, C0=None):
+         copy_mode=0, C0=None, split_k=0):
     m, k = A.shape
     n = B.shape[1]
     dC = to_dev(C0) if C0 is not None else empty(m, n, fill=0.0)
@@ -45,7 +47,7 @@ def _run(mgx, A, B, panel, *, pinned, form=0, slices=8, graph=False, serialize=F
         try:
             st = mgx.summa_run(pa.ptr, k, pb.ptr, n, dC, n, m=m, n=n, k=k, panel=panel, form=form,
                                slices=slices, use_graph=graph, serialize=serialize,
-                               copy_mode=copy_mode)
+                               copy_mode=copy_mode, split_k=split_k)
         finally:
             pa.free()
             pb.free()
@@ -53,7 +55,8 @@ def _run(mgx, A, B, panel, *, pinned, form=0, slices=8, graph=False, serialize=F
         A = np.ascontiguousarray(A)
         B = np.ascontiguousarray(B)
         st = mgx.summa_run(A.ctypes.data, k, B.ctypes.data, n, dC, n, m=m, n=n, k=k, panel=panel,
-                           form=form, slices=slices, serialize=serialize, copy_mode=copy_mode)
+                           form=form, slices=slices, serialize=serialize, copy_mode=copy_mode,
+                           split_k=split_k)
     assert st["panels"] == (k + min(panel, k) - 1) // min(panel, k)
     assert st["host_pinned"] == (1 if pinned else 0)
     return to_host(dC), st
@@ -115,34 +118,54 @@ def test_graph_replays_are_identical(mgx, orc):
         assert np.array_equal(first, again)
 
 
-def test_int8_form_panels_bit_exact(mgx, orc):
-    m, n, k, panel = 190, 170, 600, 256
-    announce("summa int8 form")
+def _grouped(A, B, g, product):
+    """The oracle's split product per group of g columns, accumulated into C
+    in group order."""
+    k = A.shape[1]
+    ref = np.zeros((A.shape[0], B.shape[1]))
+    for g0 in range(0, k, g):
+        ref = product(np.ascontiguousarray(A[:, g0:g0 + g]), np.ascontiguousarray(B[g0:g0 + g]),
+                      C=ref)
+    return ref
+
+
+@pytest.mark.parametrize("split_k,group", [(0, 600), (256, 256), (300, 384), (100, 128)])
+@pytest.mark.parametrize("form_name", ["int8", "mod"])
+def test_split_forms_multiply_whole_panel_groups(mgx, orc, form_name, split_k, group):
+    # panels of 128 (ragged last of 88); each group of whole panels is one
+    # split product, bit-exact against the oracle's model at that depth
+    m, n, k, panel = 190, 170, 600, 128
+    announce(f"summa {form_name} split_k={split_k}")
+    form = mgx.FORM_INT8 if form_name == "int8" else mgx.FORM_INT8_MOD
+    assert mgx.split_group_k(form, panel, k, split_k) == group
+    assert mgx.split_group_k(mgx.FORM_FP64, panel, k, split_k) == panel
     A = orc.fill(m, k, 1, seed=41)
     B = orc.fill(k, n, 1, seed=42)
-    C, _ = _run(mgx, A, B, panel, pinned=True, form=mgx.FORM_INT8, slices=8)
-    ref = np.zeros((m, n))
-    for k0 in range(0, k, panel):
-        kc = min(panel, k - k0)
-        ref = orc.split_gemm_int8(np.ascontiguousarray(A[:, k0:k0 + kc]),
-                                  np.ascontiguousarray(B[k0:k0 + kc]), 8, C=ref)
+    if form_name == "int8":
+        C, st = _run(mgx, A, B, panel, pinned=True, form=form, slices=8, split_k=split_k)
+        ref = _grouped(A, B, group, lambda a, b, C: orc.split_gemm_int8(a, b, 8, C=C))
+    else:
+        C, st = _run(mgx, A, B, panel, pinned=True, form=form, slices=15, split_k=split_k)
+        ref = _grouped(A, B, group, lambda a, b, C: orc.crt_gemm(a, b, 15, C=C))
+    assert st["panels"] == 5
     assert np.array_equal(C, ref)
 
 
-def test_modular_form_panels_bit_exact(mgx, orc):
-    # each panel product is the oracle's modular model at that panel depth,
-    # accumulated into C in panel order
-    m, n, k, panel = 190, 170, 600, 256
-    announce("summa modular form")
-    A = orc.fill(m, k, 1, seed=43)
-    B = orc.fill(k, n, 1, seed=44)
-    C, _ = _run(mgx, A, B, panel, pinned=True, form=mgx.FORM_INT8_MOD, slices=15)
-    ref = np.zeros((m, n))
-    for k0 in range(0, k, panel):
-        kc = min(panel, k - k0)
-        ref = orc.crt_gemm(np.ascontiguousarray(A[:, k0:k0 + kc]),
-                           np.ascontiguousarray(B[k0:k0 + kc]), 15, C=ref)
-    assert np.array_equal(C, ref)
+def test_split_groups_agree_across_streaming_modes(mgx, orc):
+    # groups of 2 panels, ragged last panel inside the last group; pinned DMA,
+    # pageable staging, host-synchronised copies, serialized and graph runs
+    m, n, k, panel = 160, 150, 1000, 128
+    announce("summa modular groups: streaming modes")
+    A = orc.fill(m, k, 1, seed=45)
+    B = orc.fill(k, n, 1, seed=46)
+    kw = dict(form=mgx.FORM_INT8_MOD, slices=15, split_k=256)
+    C0, _ = _run(mgx, A, B, panel, pinned=True, **kw)
+    assert np.array_equal(C0, _grouped(A, B, 256, lambda a, b, C: orc.crt_gemm(a, b, 15, C=C)))
+    for extra in (dict(pinned=False), dict(pinned=True, graph=True),
+                  dict(pinned=True, serialize=True), dict(pinned=True, copy_mode=1),
+                  dict(pinned=False, copy_mode=1)):
+        C1, _ = _run(mgx, A, B, panel, **extra, **kw)
+        assert np.array_equal(C0, C1), extra
 
 
 def test_invalid_summa_arguments(mgx):
