@@ -71,6 +71,24 @@ __host__ __device__ inline int residue_int8(double x, int l) {
     return r;
 }
 
+// As residue_int8 for |x| <= 2^51 (beta <= 51).  The quotient estimate
+// rint(x * fl(1/m)) then misses rint(x / m) only where |r| would exceed m/2,
+// and by at most |x| 2^-52 (1 + 2^-54) / m < 1/2 + 2^-55 in units of m, so
+// |r| <= m/2 + 1/2: within int8 for m <= 253, and only r = 128 (m = 255)
+// needs folding.  The low byte of x (m = 256) comes from one FP64 add:
+// 1.5 * 2^52 + x is exact and its low mantissa bits are x mod 2^32.
+__host__ __device__ inline int residue_int8_narrow(double x, int l) {
+    if (l == 0) {
+        const uint64_t bits = __builtin_bit_cast(uint64_t, x + 6755399441055744.0);
+        return (int)(int8_t)(uint8_t)(bits & 0xffu);
+    }
+    const double m = (double)MODULI[l];
+    const double q = rint(x * (1.0 / m));
+    int r = (int)fma(-q, m, x);
+    if (l == 1 && r > 127) r -= (int)MODULI[1];
+    return r;
+}
+
 // Correctly rounded (to nearest, ties to even) conversion of the 128-bit
 // magnitude hi:lo to double.
 __host__ __device__ inline double u128_to_double_rn(uint64_t hi, uint64_t lo) {

@@ -33,7 +33,7 @@
 //     INT32 (reduced mod m_l between K chunks of at most 2046 tiles) and
 //     stores it mod m_l as one byte per output.
 //  3. A reconstruction kernel recovers the integer C' = A'B' from its n
-//     residues (Garner), rounds it once to FP64 and writes
+//     residues (direct Chinese remaindering), rounds it once to FP64 and writes
 //     C = ldexp(C', e_i + f_j - 2 beta) (or C_in + that).
 // The modular form needs n products instead of s(s+1)/2 and its only error
 // is the quantisation of A and B to beta bits.
@@ -183,15 +183,18 @@ __device__ inline void emit_slices16(const double *x, int s, int8_t *dst, int64_
 // Writes the n residues (modular form) of 16 consecutive integer-valued
 // scaled values x[0..15] as 16 bytes each.  dst points at modulus 0;
 // consecutive moduli are `plane_stride` apart.
+template <bool NARROW>
 __device__ inline void emit_residues16(const double *x, int n, int8_t *dst, int64_t plane_stride) {
 #pragma unroll
     for (int l = 0; l < MAX_MODULI; ++l) {
         if (l < n) {
             unsigned int words[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-            for (int i = 0; i < 16; ++i)
-                words[i / 4] |= ((unsigned int)(mgx::modular::residue_int8(x[i], l) & 0xff))
-                                << (8 * (i % 4));
+            for (int i = 0; i < 16; ++i) {
+                const int r = NARROW ? mgx::modular::residue_int8_narrow(x[i], l)
+                                     : mgx::modular::residue_int8(x[i], l);
+                words[i / 4] |= ((unsigned int)(r & 0xff)) << (8 * (i % 4));
+            }
             const v4i packed = {(int)words[0], (int)words[1], (int)words[2], (int)words[3]};
             *(v4i *)(dst + (int64_t)l * plane_stride) = packed;
         }
@@ -258,12 +261,17 @@ __global__ __launch_bounds__(256) void operand_tile_kernel(const double *__restr
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const double a = COLS ? tile[(kq * 16 + i) * STR + line] : tile[line * STR + kq * 16 + i];
-        x[i] = e != EXP_BAD ? scaled_operand<MOD>(a, e, beta) : 0.0;
+        const double v = scaled_operand<MOD>(a, e != EXP_BAD ? e : 0, beta);
+        x[i] = e != EXP_BAD ? v : 0.0;
     }
     int8_t *dst = Xop + (kt * Lp + gl) * KT + kq * 16;
-    if (MOD)
-        emit_residues16(x, planes, dst, KTC * Lp * KT);
-    else
+    if (MOD) {
+        // |A'| <= 2^beta: beta <= 51 takes the shorter residue sequence
+        if (beta <= 51)
+            emit_residues16<true>(x, planes, dst, KTC * Lp * KT);
+        else
+            emit_residues16<false>(x, planes, dst, KTC * Lp * KT);
+    } else
         emit_slices16(x, planes, dst, KTC * Lp * KT);
 }
 

@@ -118,6 +118,37 @@ def test_long_depth_reduces_between_chunks(mgx, orc):
     assert _same(_mod(mgx, A, B, 16), orc.crt_gemm(A, B, 16))
 
 
+@pytest.mark.parametrize("k,beta", [(16384, 51), (4096, 52)])
+def test_residues_at_the_top_of_the_operand_range(mgx, orc, k, beta):
+    # integer operands with |A'| up to 2^beta - 1 (row and column maxima in
+    # [2^(beta-1), 2^beta), so A' = A): beta = 51 is the widest depth of the
+    # short residue sequence, 52 takes the general one.  Entries sit next to
+    # the half-way points j m + m/2 of every odd modulus, where the quotient
+    # estimate can miss by one.
+    assert orc.crt_beta(15, k) == beta
+    m, n = 24, 20
+    announce(f"split mod operand range edge k={k} beta={beta}")
+    r = np.random.default_rng(beta)
+    top = 2 ** beta
+    mods = np.array([orc.crt_modulus(l) for l in range(1, 15)], dtype=np.int64)
+
+    def edge(rows, cols):
+        size = rows * cols
+        mm = mods[r.integers(0, len(mods), size)]
+        j = (top - 1) // mm - r.integers(0, 1000, size)
+        v = np.minimum(j * mm + mm // 2 + r.integers(-2, 3, size), top - 1)
+        return (v * r.choice(np.array([-1, 1]), size)).reshape(rows, cols)
+
+    Ai = edge(m, k)
+    Bi = edge(k, n)
+    Ai[:, 0] = top - 1          # every row / column maximum in [2^(beta-1), 2^beta)
+    Bi[0, :] = -(top - 1)
+    A = Ai.astype(np.float64)
+    B = Bi.astype(np.float64)
+    assert np.array_equal(A.astype(np.int64), Ai) and np.array_equal(B.astype(np.int64), Bi)
+    assert _same(_mod(mgx, A, B, 15), orc.crt_gemm(A, B, 15))
+
+
 def test_beta_matches_model_and_errors(mgx, orc):
     import ctypes
     for nmod in (2, 8, 14, 15, 16):

@@ -52,7 +52,8 @@ def test_split_tiles_bit_exact(mgx, orc, tile, m, n, k, s):
 
 
 def test_time_local_product_result_and_timing(mgx, orc):
-    # The timed panel loop leaves C = A.B (first panel overwrites, later ones add).
+    # The timed panel loop leaves C = A.B (first panel or panel group
+    # overwrites, later ones add), for every form and split group.
     r = np.random.default_rng(5)
     m, n, k = 256, 192, 1000
     announce("time_local_product")
@@ -60,10 +61,10 @@ def test_time_local_product_result_and_timing(mgx, orc):
     B = r.integers(-30, 30, size=(k, n)).astype(np.float64)
     exact = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float64)
     dA, dB = to_dev(A), to_dev(B)
-    for form, slices in ((0, 0), (1, 8)):
+    for form, slices, split_k in ((0, 0, 0), (1, 8, 0), (1, 8, 300), (3, 15, 0), (3, 15, 300)):
         for tile in (1, 2):
             dC = empty(m, n)
-            cfg = mgx._lib.TuneConfig(form, slices, tile, 256)
+            cfg = mgx._lib.TuneConfig(form, slices, tile, 256, split_k)
             ms = ctypes.c_double()
             mgx._lib.call("mgx_time_local_product", ctypes.byref(cfg), dA.data_ptr(),
                           dB.data_ptr(), dC.data_ptr(), m, n, k, k, n, n, None, 0, 1, 2,
