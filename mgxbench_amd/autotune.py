@@ -171,8 +171,10 @@ class Timer:
         return ms.value
 
 
-def measure(shapes, out_path: Path, log=print) -> dict:
-    """Times every candidate of every shape; resumes from out_path if present."""
+def measure(shapes, out_path: Path, log=print, time_budget_s: float | None = None) -> dict:
+    """Times every candidate of every shape; resumes from out_path if present.
+    With a time budget, stops before a shape that would likely overrun it
+    (the dataset stays checkpointed; a later call resumes)."""
     doc = {"schema": SCHEMA, "kind": "mgx-autotune-dataset", "rows": [], "shapes_done": []}
     if out_path.exists():
         doc = json.loads(out_path.read_text())
@@ -184,8 +186,13 @@ def measure(shapes, out_path: Path, log=print) -> dict:
     nn = max(s[1] for s in todo)
     kk = max(s[2] for s in todo)
     timer = Timer(mm, nn, kk)
+    start = time.time()
+    worst = 0.0
     for i, (m, n, k) in enumerate(todo):
         t0 = time.time()
+        if time_budget_s is not None and (t0 - start) + 1.5 * worst > time_budget_s:
+            log(f"[autotune] time budget reached after {i} of {len(todo)} shapes")
+            break
         for c in candidate_space(k):
             ms = timer.time(m, n, k, c)
             doc["rows"].append({"m": m, "n": n, "k": k, **asdict(c), "ms": ms})
@@ -193,6 +200,7 @@ def measure(shapes, out_path: Path, log=print) -> dict:
         tmp = out_path.with_suffix(".tmp")
         tmp.write_text(json.dumps(doc))
         os.replace(tmp, out_path)
+        worst = max(worst, time.time() - t0)
         log(f"[autotune] shape {i + 1}/{len(todo)} m={m} n={n} k={k} "
             f"({len(candidate_space(k))} candidates, {time.time() - t0:.1f} s)")
     return doc
@@ -423,6 +431,8 @@ def main(argv=None) -> int:
     p.add_argument("--out", required=True)
     p.add_argument("--random-shapes", type=int, default=40)
     p.add_argument("--max-size", type=int, default=32768)
+    p.add_argument("--time-budget", type=float, default=None,
+                   help="seconds; stop before a shape that would overrun it")
     p = sub.add_parser("train")
     p.add_argument("--dataset", required=True)
     p.add_argument("--out", required=True)
@@ -443,7 +453,7 @@ def main(argv=None) -> int:
     if a.cmd == "measure":
         sizes = [s for s in SIZES if s <= a.max_size]
         shapes = rank_shares(sizes) + random_shapes(a.random_shapes, sizes=sizes)
-        doc = measure(shapes, Path(a.out))
+        doc = measure(shapes, Path(a.out), time_budget_s=a.time_budget)
         print(f"[autotune] {len(doc['rows'])} measured configurations in {a.out}")
     elif a.cmd == "train":
         doc = json.loads(Path(a.dataset).read_text())
