@@ -39,8 +39,9 @@ enum {
     MGX_FORM_FP64 = 0,         /* plain FP64 MFMA (v_mfma_f64_16x16x4_f64) */
     MGX_FORM_INT8 = 1,         /* Ozaki scheme I: INT8 digit slices, exact INT32 MFMA sums of
                                   the s(s+1)/2 slice pairs, FP64 folds */
-    MGX_FORM_BF16 = 2,         /* reserved (BF16 slices); not built: every call returns
-                                  MGX_ERR_UNSUPPORTED */
+    MGX_FORM_BF16 = 2,         /* the INT8 form's digit slices held as bf16, products on the
+                                  bf16 MFMA with FP32 sums moved to INT32 while exact: same
+                                  result bit for bit as MGX_FORM_INT8, half its matrix rate */
     MGX_FORM_INT8_MOD = 3      /* Ozaki scheme II: INT8 residues of the 2^beta-scaled integer
                                   operands modulo `slices` coprime moduli, one exact INT32 MFMA
                                   product per modulus, Chinese remaindering to FP64 */
@@ -125,7 +126,10 @@ int mgx_validate_constant(const double *dC, int64_t rows, int64_t cols,
  * is ldexp(RN(A'B'), e_i + f_j - 2 beta) with A', B' the 2^beta-scaled
  * integer operands (see mgx_split_mod_beta), so each output is rounded once.
  * Its events: [2] before the residue products, [3] after them (the
- * reconstruction into C follows [3]). */
+ * reconstruction into C follows [3]).
+ * MGX_FORM_BF16: `slices` 1..9 as for MGX_FORM_INT8, same digits, group
+ * sums, chunks and folds, so the same result bit for bit; always runs the
+ * MGX_TILE_SMALL block tile (it keeps FP32 and INT32 accumulators). */
 size_t mgx_split_workspace_bytes(int form, int slices,
                                  int64_t m, int64_t n, int64_t k);
 int mgx_split_gemm(int form, int slices,

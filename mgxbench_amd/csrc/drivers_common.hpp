@@ -54,7 +54,7 @@ struct Options {
     uint64_t seed = 1;
 };
 
-// MGX_FORM=fp64|int8|mod, MGX_SLICES=<n> (slices of int8, default 8; moduli
+// MGX_FORM=fp64|int8|bf16|mod, MGX_SLICES=<n> (slices of int8 / bf16, default 8; moduli
 // of mod, default 15), MGX_INIT=const|uniform[:SEED].  Split forms also read
 // MGX_SPLIT_K (columns per split product in the panel / chunk loops, whole
 // panels or chunks; default MGX_SPLIT_GROUP_K) where the loop runs.
@@ -63,6 +63,7 @@ inline bool parse_options(Options *o) {
     if (form == "fp64") o->form = MGX_FORM_FP64;
     else if (form == "int8") o->form = MGX_FORM_INT8;
     else if (form == "mod") o->form = MGX_FORM_INT8_MOD;
+    else if (form == "bf16") o->form = MGX_FORM_BF16;
     else return false;
     o->slices = env_int("MGX_SLICES", o->form == MGX_FORM_INT8_MOD ? 15 : 8);
     const std::string init = env_str("MGX_INIT", "const");
@@ -79,7 +80,10 @@ inline bool parse_options(Options *o) {
 }
 
 inline const char *form_name(int form) {
-    return form == MGX_FORM_INT8 ? "int8" : form == MGX_FORM_INT8_MOD ? "mod" : "fp64";
+    return form == MGX_FORM_INT8       ? "int8"
+           : form == MGX_FORM_INT8_MOD ? "mod"
+           : form == MGX_FORM_BF16     ? "bf16"
+                                       : "fp64";
 }
 
 // splitmix64 counter stream, the definition used by mgx_fill.

@@ -57,6 +57,8 @@ constexpr long long SLICE_MAX_PRODUCT = 128ll * 128ll;  // largest |sigma_p sigm
 constexpr int MOD_CHUNK_TILES = 2046;   // 2046 * 64 * 128^2 + 256 < 2^31
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 inline int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
@@ -180,6 +182,39 @@ __device__ inline void emit_slices16(const double *x, int s, int8_t *dst, int64_
     }
 }
 
+// BF16 digit planes: the same digits as bf16 values (every integer of
+// magnitude <= 256 is exact in bf16's 8 significant bits), 16 consecutive
+// values as 32 bytes.  dst points at slice 1; consecutive slices are
+// `slice_stride` apart.
+__device__ inline void emit_slices16_bf16(const double *x, int s, int8_t *dst,
+                                          int64_t slice_stride) {
+    unsigned int words[MAX_SLICES][8];
+#pragma unroll
+    for (int p = 0; p < MAX_SLICES; ++p)
+#pragma unroll
+        for (int w = 0; w < 8; ++w) words[p][w] = 0u;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        int dig[MAX_SLICES];
+        slice_digits(x[i], s, dig);
+#pragma unroll
+        for (int p = 0; p < MAX_SLICES; ++p)
+            if (p < s) {
+                // bf16 = the upper half of the (exact) fp32 pattern
+                const unsigned int h = __float_as_uint((float)dig[p]) >> 16;
+                words[p][i / 2] |= h << (16 * (i % 2));
+            }
+    }
+#pragma unroll
+    for (int p = 0; p < MAX_SLICES; ++p) {
+        if (p < s) {
+            v4i *d = (v4i *)(dst + (int64_t)p * slice_stride);
+            d[0] = v4i{(int)words[p][0], (int)words[p][1], (int)words[p][2], (int)words[p][3]};
+            d[1] = v4i{(int)words[p][4], (int)words[p][5], (int)words[p][6], (int)words[p][7]};
+        }
+    }
+}
+
 // Writes the n residues (modular form) of 16 consecutive integer-valued
 // scaled values x[0..15] as 16 bytes each.  dst points at modulus 0;
 // consecutive moduli are `plane_stride` apart.
@@ -226,7 +261,11 @@ __device__ inline double scaled_operand(double a, int e, int beta) {
 // bytes per plane; the 64 lanes of a wave write one contiguous KiB per plane.
 // LDS rows are padded to 528 bytes so that both read patterns are free of
 // bank conflicts.
-template <bool MOD, bool COLS>
+//
+// BF (digit slices only): bf16 planes, 2 bytes per value, so a 64-byte plane
+// row holds 32 k values and the plane has KTC = 2 Kp / 64 row blocks; the 16
+// values of a lane are 32 contiguous bytes of one row.
+template <bool MOD, bool COLS, bool BF = false>
 __global__ __launch_bounds__(256) void operand_tile_kernel(const double *__restrict__ X,
                                                            int64_t lines, int64_t k, int64_t ld,
                                                            const int32_t *__restrict__ ex,
@@ -263,6 +302,13 @@ __global__ __launch_bounds__(256) void operand_tile_kernel(const double *__restr
         const double a = COLS ? tile[(kq * 16 + i) * STR + line] : tile[line * STR + kq * 16 + i];
         const double v = scaled_operand<MOD>(a, e != EXP_BAD ? e : 0, beta);
         x[i] = e != EXP_BAD ? v : 0.0;
+    }
+    if (BF) {
+        // k values 16 kq .. 16 kq + 15 of this 64-k tile: bf16 row block
+        // 2 kt + kq / 2, bytes 32 (kq % 2) .. +32
+        int8_t *dst = Xop + ((2 * kt + (kq >> 1)) * Lp + gl) * KT + (kq & 1) * 32;
+        emit_slices16_bf16(x, planes, dst, KTC * Lp * KT);
+        return;
     }
     int8_t *dst = Xop + (kt * Lp + gl) * KT + kq * 16;
     if (MOD) {
@@ -317,7 +363,7 @@ __device__ inline int chunk_tiles(int np) {
 // and pair changes take the slow path.  Past the last step the cursor stays
 // on the last stage, so the load that keeps the per-step load count uniform
 // re-reads a valid tile into a slot no step reads again.
-template <bool MOD>
+template <bool MOD, bool BF = false>
 struct LoadCursor {
     const int8_t *ga, *gb;   // tile base of the current stage (no lane offset)
     int kt, c1;              // current k tile, end of the current chunk
@@ -337,7 +383,8 @@ struct LoadCursor {
     __device__ void begin_group() {
         p = first_pair();
         pmax = MOD ? t : min(s, t - 1);
-        chunk = MOD ? MOD_CHUNK_TILES : chunk_tiles(pmax - p + 1);
+        // bf16 planes hold 32 k values per 64-byte tile: the same k span
+        chunk = MOD ? MOD_CHUNK_TILES : (BF ? 2 : 1) * chunk_tiles(pmax - p + 1);
         c0 = 0;
         c1 = min(chunk, ktiles);
         kt = 0;
@@ -521,8 +568,18 @@ struct ProductArgs {
     int accumulate;
 };
 
-template <int TB, int WGM, int WGN, bool MOD, int MINB>
-__global__ __launch_bounds__(64 * WGM * WGN, MINB) void int8_product_kernel(ProductArgs a) {
+// The product kernels' body.  BF (digit slices only) runs the bf16 planes on
+// the bf16 MFMA (16x16x32, FP32 accumulation): same bytes per fragment and
+// per LDS stage as INT8, 32 k values per 64-byte tile.  The FP32 sums are
+// moved into the INT32 accumulators every BF_FLUSH_STEPS steps, while they
+// are still exact integers (16 steps x 32 products x 2^14 = 2^23 < 2^24), so
+// the group sums, chunks and folds are those of the INT8 digit-slice form and
+// the result is bit-identical to it.
+constexpr int BF_FLUSH_STEPS = 16;
+
+template <int TB, int WGM, int WGN, bool MOD, bool BF>
+__device__ __forceinline__ void product_body(const ProductArgs &a) {
+    static_assert(!(MOD && BF), "bf16 planes carry digit slices only");
     using S = TileShape<TB, WGM, WGN>;
     constexpr int FI = S::FI, FJ = S::FJ, TILE = S::TILE, SLOT = S::SLOT, LPW = S::LPW;
     // operand ring
@@ -566,10 +623,11 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINB) void int8_product_kernel(Prod
 
     v4i acc[FI][FJ];
     const v4i vzero = {0, 0, 0, 0};
+    v4f accf[BF ? FI : 1][BF ? FJ : 1];   // bf16 form: FP32 partial sums
     constexpr int LPS = 2 * LPW;   // load instructions per wave per stage
 
     const int ktiles = (int)a.KTC;
-    LoadCursor<MOD> cur;
+    LoadCursor<MOD, BF> cur;
     cur.start(s, ktiles, a.A + m0 * KT, a.B + n0 * KT, a.Mp * KT, a.Np * KT, a.KTC * a.Mp * KT,
               a.KTC * a.Np * KT);
     const uint32_t voff = (uint32_t)lane_off;
@@ -627,9 +685,15 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINB) void int8_product_kernel(Prod
 #pragma unroll
         for (int i = 0; i < FH; ++i)
 #pragma unroll
-            for (int j = 0; j < FJ; ++j)
-                acc[i0 + i][j] =
-                    __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i0 + i][j], 0, 0, 0);
+            for (int j = 0; j < FJ; ++j) {
+                if constexpr (BF)
+                    accf[i0 + i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(v8bf, fa[i]), __builtin_bit_cast(v8bf, fb[j]),
+                        accf[i0 + i][j], 0, 0, 0);
+                else
+                    acc[i0 + i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j],
+                                                                           acc[i0 + i][j], 0, 0, 0);
+            }
     };
     // Software pipeline: the next step is published and its fragments are
     // read between the two halves of the current step's MFMAs, so barrier
@@ -667,14 +731,31 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINB) void int8_product_kernel(Prod
         bool first_fold = true;
         for (int t = 2; t <= s + 1; ++t) {
             const int np = min(s, t - 1) - max(1, t - s) + 1;
-            const int chunk = chunk_tiles(np);
+            const int chunk = (BF ? 2 : 1) * chunk_tiles(np);
             for (int c0 = 0; c0 < ktiles; c0 += chunk) {
 #pragma unroll
                 for (int i = 0; i < FI; ++i)
 #pragma unroll
                     for (int j = 0; j < FJ; ++j) acc[i][j] = vzero;
                 // chunks and KTC are even, so every chunk has an even step count
-                run_steps(min(chunk, ktiles - c0) * np);
+                if constexpr (BF) {
+                    for (int left = min(chunk, ktiles - c0) * np; left > 0;
+                         left -= BF_FLUSH_STEPS) {
+#pragma unroll
+                        for (int i = 0; i < FI; ++i)
+#pragma unroll
+                            for (int j = 0; j < FJ; ++j) accf[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+                        run_steps(min(left, BF_FLUSH_STEPS));
+#pragma unroll
+                        for (int i = 0; i < FI; ++i)
+#pragma unroll
+                            for (int j = 0; j < FJ; ++j)
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) acc[i][j][r] += (int)accf[i][j][r];
+                    }
+                } else {
+                    run_steps(min(chunk, ktiles - c0) * np);
+                }
                 const int64_t row0 = m0 + wm * S::WTM + 4 * lk;
                 const int64_t col0 = n0 + wn * S::WTN + lc;
                 const bool overwrite = first_fold && !a.accumulate;
@@ -720,6 +801,16 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINB) void int8_product_kernel(Prod
             wait_vmcnt<0>();
         }
     }
+}
+
+template <int TB, int WGM, int WGN, bool MOD, int MINB>
+__global__ __launch_bounds__(64 * WGM * WGN, MINB) void int8_product_kernel(ProductArgs a) {
+    product_body<TB, WGM, WGN, MOD, false>(a);
+}
+
+template <int TB, int WGM, int WGN, int MINB>
+__global__ __launch_bounds__(64 * WGM * WGN, MINB) void bf16_product_kernel(ProductArgs a) {
+    product_body<TB, WGM, WGN, false, true>(a);
 }
 
 // Scheme II reconstruction: one thread per (block tile, wave, fragment
@@ -798,7 +889,8 @@ unsigned grid1d(int64_t total, int threads, int64_t cap = 65536) {
 }
 
 bool form_planes_ok(int form, int planes) {
-    if (form == MGX_FORM_INT8) return planes >= MIN_SLICES && planes <= MAX_SLICES;
+    if (form == MGX_FORM_INT8 || form == MGX_FORM_BF16)
+        return planes >= MIN_SLICES && planes <= MAX_SLICES;
     if (form == MGX_FORM_INT8_MOD) return planes >= MIN_MODULI && planes <= MAX_MODULI;
     return false;
 }
@@ -817,11 +909,13 @@ SplitLayout split_layout(int form, int planes, int64_t m, int64_t n, int64_t k) 
         off += (bytes + 255) / 256 * 256;
         return o;
     };
-    L.KTC = L.Kp / KT;
+    // bytes per plane value: bf16 planes take two
+    const int64_t esize = form == MGX_FORM_BF16 ? 2 : 1;
+    L.KTC = L.Kp * esize / KT;
     L.Mp = round_up(m < 1 ? 1 : m, ROW_ALIGN);
     L.Np = round_up(n < 1 ? 1 : n, ROW_ALIGN);
-    L.a_off = take((size_t)planes * L.Mp * L.Kp);
-    L.b_off = take((size_t)planes * L.Np * L.Kp);
+    L.a_off = take((size_t)planes * L.Mp * L.Kp * esize);
+    L.b_off = take((size_t)planes * L.Np * L.Kp * esize);
     // product residues: every block tile of either tile size fits in Mp x Np
     L.r_off = take(form == MGX_FORM_INT8_MOD ? (size_t)planes * L.Mp * L.Np : 0);
     L.rmax_off = take((size_t)(m + n) * 8);
@@ -837,7 +931,7 @@ SplitLayout split_layout(int form, int planes, int64_t m, int64_t n, int64_t k) 
 extern "C" int mgx_split_slices_range(int form, int *min_slices, int *max_slices) {
     mgx::clear_error();
     int lo, hi;
-    if (form == MGX_FORM_INT8) {
+    if (form == MGX_FORM_INT8 || form == MGX_FORM_BF16) {
         lo = MIN_SLICES;
         hi = MAX_SLICES;
     } else if (form == MGX_FORM_INT8_MOD) {
@@ -869,14 +963,17 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     if (tile == MGX_TILE_DEFAULT) tile = MGX_TILE_LARGE;
     if (tile != MGX_TILE_LARGE && tile != MGX_TILE_SMALL)
         return mgx::set_error(MGX_ERR_UNSUPPORTED, "unknown tile");
-    const int64_t TBsz = tile == MGX_TILE_LARGE ? 256 : 128;
-    if (form != MGX_FORM_INT8 && form != MGX_FORM_INT8_MOD)
+    if (form != MGX_FORM_INT8 && form != MGX_FORM_INT8_MOD && form != MGX_FORM_BF16)
         return mgx::set_error(MGX_ERR_UNSUPPORTED, "split form not supported");
     if (!form_planes_ok(form, slices))
-        return mgx::set_error(MGX_ERR_UNSUPPORTED, form == MGX_FORM_INT8
-                                                       ? "slice count out of range"
-                                                       : "modulus count out of range");
+        return mgx::set_error(MGX_ERR_UNSUPPORTED, form == MGX_FORM_INT8_MOD
+                                                       ? "modulus count out of range"
+                                                       : "slice count out of range");
     const bool mod = form == MGX_FORM_INT8_MOD;
+    // the bf16 form keeps FP32 and INT32 accumulators: the 128 tile only
+    const bool bf = form == MGX_FORM_BF16;
+    if (bf) tile = MGX_TILE_SMALL;
+    const int64_t TBsz = tile == MGX_TILE_LARGE ? 256 : 128;
     MGX_ARG_CHECK(m >= 0 && n >= 0 && k >= 0, "negative matrix dimension");
     if (m == 0 || n == 0) return MGX_OK;
     MGX_ARG_CHECK(C != nullptr && ldc >= n, "C is NULL or ldc < n");
@@ -946,8 +1043,12 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     {
         const int64_t gy = L.Mp / 64;
         MGX_ARG_CHECK(gy <= 65535 && L.KTC <= 0x7fffffffll, "m or k too large for the row pass");
-        const dim3 grid((unsigned)L.KTC, (unsigned)gy);
-        if (mod)
+        // one workgroup per 64 k values and 64 lines
+        const dim3 grid((unsigned)(L.Kp / KT), (unsigned)gy);
+        if (bf)
+            hipLaunchKernelGGL((operand_tile_kernel<false, false, true>), grid, dim3(256), 0, st,
+                               A, m, k, lda, rexp, slices, 0, L.Mp, L.KTC, Aop);
+        else if (mod)
             hipLaunchKernelGGL((operand_tile_kernel<true, false>), grid, dim3(256), 0, st, A, m, k,
                                lda, rexp, slices, beta, L.Mp, L.KTC, Aop);
         else
@@ -958,8 +1059,11 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     {
         const int64_t gy = L.Np / 64;
         MGX_ARG_CHECK(gy <= 65535, "n too large for the column pass");
-        const dim3 grid((unsigned)L.KTC, (unsigned)gy);
-        if (mod)
+        const dim3 grid((unsigned)(L.Kp / KT), (unsigned)gy);
+        if (bf)
+            hipLaunchKernelGGL((operand_tile_kernel<false, true, true>), grid, dim3(256), 0, st,
+                               B, n, k, ldb, cexp, slices, 0, L.Np, L.KTC, Bop);
+        else if (mod)
             hipLaunchKernelGGL((operand_tile_kernel<true, true>), grid, dim3(256), 0, st, B, n, k,
                                ldb, cexp, slices, beta, L.Np, L.KTC, Bop);
         else
@@ -983,7 +1087,10 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     pa.KTC = L.KTC;
     pa.s = slices;
     pa.accumulate = accumulate ? 1 : 0;
-    if (tile == MGX_TILE_LARGE) {
+    if (bf) {
+        hipLaunchKernelGGL((bf16_product_kernel<128, 2, 2, 2>), dim3((unsigned)tiles),
+                           dim3(ShapeSmall::THREADS), 0, st, pa);
+    } else if (tile == MGX_TILE_LARGE) {
         if (mod)
             hipLaunchKernelGGL((int8_product_kernel<256, 2, 4, true, 1>), dim3((unsigned)tiles),
                                dim3(ShapeLarge::THREADS), 0, st, pa);

@@ -82,7 +82,9 @@ bool host_is_pinned(const void *p) {
 
 extern "C" int64_t mgx_split_group_k(int form, int64_t panel, int64_t split_k, int64_t k) {
     if (panel <= 0 || k <= 0) return -1;
-    if (form != MGX_FORM_FP64 && form != MGX_FORM_INT8 && form != MGX_FORM_INT8_MOD) return -1;
+    if (form != MGX_FORM_FP64 && form != MGX_FORM_INT8 && form != MGX_FORM_INT8_MOD &&
+        form != MGX_FORM_BF16)
+        return -1;
     const int64_t p = panel < k ? panel : k;
     if (form == MGX_FORM_FP64) return p;
     const int64_t want = split_k > 0 ? split_k : MGX_SPLIT_GROUP_K;
@@ -100,7 +102,7 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
     MGX_ARG_CHECK(hA && hB && dC, "NULL matrix pointer");
     MGX_ARG_CHECK(lda >= k && ldb >= n && ldc >= n, "leading dimension too small");
     MGX_ARG_CHECK(P->form == MGX_FORM_FP64 || P->form == MGX_FORM_INT8 ||
-                      P->form == MGX_FORM_INT8_MOD,
+                      P->form == MGX_FORM_INT8_MOD || P->form == MGX_FORM_BF16,
                   "unsupported form");
     MGX_ARG_CHECK(P->copy_mode == MGX_SUMMA_COPY_ASYNC || P->copy_mode == MGX_SUMMA_COPY_HOST_SYNC,
                   "unknown copy mode");

@@ -35,7 +35,7 @@ extern "C" int mgx_time_local_product(const mgx_tune_config *cfg, const double *
     mgx::clear_error();
     MGX_ARG_CHECK(cfg && ms_out, "NULL pointer");
     MGX_ARG_CHECK(cfg->form == MGX_FORM_FP64 || cfg->form == MGX_FORM_INT8 ||
-                      cfg->form == MGX_FORM_INT8_MOD,
+                      cfg->form == MGX_FORM_INT8_MOD || cfg->form == MGX_FORM_BF16,
                   "unsupported form");
     MGX_ARG_CHECK(cfg->panel > 0 && m > 0 && n > 0 && k > 0, "invalid sizes");
     MGX_ARG_CHECK(warmup >= 0 && reps > 0, "invalid repetition counts");

@@ -9,8 +9,10 @@ summary comes from rocprofv3:
   * a `rocprofv3 --kernel-trace --stats --output-format csv` directory holds
     one `<name>_kernel_stats.csv` (columns Name, Calls, TotalDurationNs,
     AverageNs, Percentage, ...);
-  * a rocpd database (`.db`, rocprofv3's default output) is summarised by
-    `rocpd summary -i <db> --format csv` and its kernel table is taken;
+  * a rocpd database (`.db`, `rocprofv3 --output-format rocpd`) is
+    summarised by `rocpd summary -i <db> --format csv`, whose
+    `<name>_kernels_summary.csv` (Name, Calls, Duration (Nsec), ...,
+    Percent (Inc), ...) is taken;
   * a CSV in either the rocprofv3 or the Nsight layout can be given directly.
 
 The CSV reader keeps the reference's tolerance rules (plot_nsys_stats.py:
@@ -65,8 +67,9 @@ def columns(header):
     """(names, kernel-name column, percentage column) of a candidate header row."""
     names = [str(c).strip() for c in header]
     name = next((c for c in names if c.lower() in NAME_COLUMNS), None)
+    # Nsight "Time (%)", rocprofv3 "Percentage", rocpd summary "Percent (Inc)"
     percent = next((c for c in names if ("time" in c.lower() and "%" in c)
-                    or c.lower() == "percentage"), None)
+                    or c.lower().startswith("percent")), None)
     return names, name, percent
 
 

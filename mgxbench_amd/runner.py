@@ -93,7 +93,7 @@ def build_parser() -> argparse.ArgumentParser:
     ap.add_argument("--auto-model", default=None)
     ap.add_argument("--auto-validate", action="store_true")
     ap.add_argument("--auto-feedback", action="store_true")
-    ap.add_argument("--form", choices=["fp64", "int8", "mod"], default=None,
+    ap.add_argument("--form", choices=["fp64", "int8", "bf16", "mod"], default=None,
                     help="product form of the executables (MGX_FORM)")
     ap.add_argument("--slices", type=int, default=None, help="INT8 slices (MGX_SLICES)")
     ap.add_argument("--init", default=None, help="const | uniform:SEED (MGX_INIT)")

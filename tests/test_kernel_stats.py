@@ -36,6 +36,14 @@ def test_rocprofv3_kernel_stats_layout():
     assert df[name].iloc[0].startswith("void (anonymous namespace)::dgemm_mfma<128, 128, false>")
 
 
+def test_rocpd_summary_layout():
+    # written by `rocpd summary -i run_results.db --format csv` on an MI355X
+    df, name, percent, discarded = ks.read_summary(GOLD / "rocpd_kernels_summary.csv")
+    assert (name, percent, discarded) == ("Name", "Percent (Inc)", 0)
+    assert df[percent].sum() == pytest.approx(100.0, abs=1e-6)
+    assert df[name].iloc[0].startswith("void (anonymous namespace)::dgemm_mfma<128, 128, false>")
+
+
 def test_table_ends_at_blank_line_or_column_change(tmp_path):
     path = tmp_path / "k.csv"
     path.write_text("Name,Percentage\na,60\nb,40\n\nc,5\n")
@@ -121,8 +129,10 @@ def test_export_from_a_database_with_simulated_rocpd(tmp_path):
         "import sys\nfrom pathlib import Path\na = sys.argv\n"
         "assert a[1] == 'summary' and '--format' in a and a[a.index('--format') + 1] == 'csv'\n"
         "out = Path(a[a.index('--output-path') + 1])\n"
-        "(out / 'KERNEL_DISPATCH_summary.csv').write_text('Name,Calls,Percentage\\nk1,2,75\\nk2,1,25\\n')\n"
-        "(out / 'HIP_API_summary.csv').write_text('Name,Percentage\\nhipLaunchKernel,100\\n')\n"))
+        "stem = a[a.index('--output-file') + 1]\n"
+        "(out / (stem + '_kernels_summary.csv')).write_text("
+        "'Name,Calls,Duration (Nsec),Percent (Inc)\\nk1,2,30,75\\nk2,1,10,25\\n')\n"
+        "(out / (stem + '_hip_api_summary.csv')).write_text('Name,Percent (Inc)\\nhipLaunchKernel,100\\n')\n"))
     dest, kind = ks.export_stats(db, tmp_path / "out", rocpd=str(fake))
     assert kind == "rocpd_summary" and dest.exists()
     assert ks.read_summary(dest)[0]["Name"].tolist() == ["k1", "k2"]
