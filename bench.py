@@ -35,6 +35,15 @@ import time
 METRIC = "FP64-accurate TFLOP/s at N=16384: split-precision MFMA vs native FP64 MFMA"
 
 
+def workload_name(m: int, n: int, k: int) -> str:
+    """The campaign share these dimensions are, if any."""
+    if (m, n, k) == (8192, 8192, 16384):
+        return "2D SUMMA 2x2 rank share at N=16384"
+    if (m, n, k) == (8192, 32768, 32768):
+        return "1D row-block rank share at P=4, N=32768"
+    return "local product"
+
+
 def parse_args():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,7 +277,7 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": (f"2D SUMMA 2x2 rank share at N=16384: C {m}x{n} += A {m}x{k} . "
+            "workload": (f"{workload_name(m, n, k)}: C {m}x{n} += A {m}x{k} . "
                          f"B {k}x{n}, FP64 uniform[-1,1), HBM-resident"),
             "m": m, "n": n, "k": k,
             "form": (f"int8 modular split (Ozaki scheme II), {args.moduli} moduli, "
