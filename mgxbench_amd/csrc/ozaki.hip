@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) void row_absmax_kernel(const double *__restric
 #pragma unroll 4
         for (int i = 0; i < 16; ++i) {
             const int64_t kk = kbase + i * 256 + threadIdx.x;
-            if (kk < k) mx = max(mx, abs_bits(row[kk]));
+            if (kk < k) mx = max(mx, abs_bits(__builtin_nontemporal_load(&row[kk])));
         }
         for (int off = 32; off > 0; off >>= 1) mx = max(mx, (unsigned long long)__shfl_xor(mx, off));
         if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
@@ -114,7 +114,9 @@ __global__ __launch_bounds__(256) void col_absmax_kernel(const double *__restric
     const int64_t k0 = (int64_t)blockIdx.y * 256;
     const int64_t k1 = min(k, k0 + 256);
     unsigned long long mx = 0;
-    for (int64_t kk = k0; kk < k1; ++kk) mx = max(mx, abs_bits(B[kk * ldb + c]));
+    // read-once stream: non-temporal loads
+    for (int64_t kk = k0; kk < k1; ++kk)
+        mx = max(mx, abs_bits(__builtin_nontemporal_load(&B[kk * ldb + c])));
     if (mx) atomicMax(colmax + c, mx);
 }
 
@@ -277,18 +279,36 @@ __global__ __launch_bounds__(256) void operand_tile_kernel(const double *__restr
     const int64_t l0 = (int64_t)blockIdx.y * 64;
     const int64_t k0 = kt * KT;
     const int t = threadIdx.x;
-    // stage: element e = t + 256 i is (source row e / 64, source column e % 64)
+    // Stage the 64 x 64 source tile (source rows are lines for A, k for B).
+    // Interior tiles of a 16-byte aligned matrix with an even leading
+    // dimension take 16-byte loads: element pair e = 2 (t + 256 i) is (source
+    // row e / 64, source columns e % 64, +1); the source is read once, so
+    // the loads are non-temporal.  Other tiles load element by element.
+    const bool full = l0 + 64 <= lines && k0 + 64 <= k;
+    const bool pairs = ((reinterpret_cast<uintptr_t>(X) & 15) == 0) && ((ld & 1) == 0);
+    if (full && pairs) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int e = t + 256 * i;
-        const int sr = e >> 6, sc = e & 63;
-        double v = 0.0;
-        if (!COLS) {   // source row = line, source column = k
-            if (l0 + sr < lines && k0 + sc < k) v = X[(l0 + sr) * ld + k0 + sc];
-        } else {       // source row = k, source column = line
-            if (k0 + sr < k && l0 + sc < lines) v = X[(k0 + sr) * ld + l0 + sc];
+        for (int i = 0; i < 8; ++i) {
+            const int e = 2 * (t + 256 * i);
+            const int sr = e >> 6, sc = e & 63;
+            const d2 *src = (const d2 *)(COLS ? &X[(k0 + sr) * ld + l0 + sc]
+                                              : &X[(l0 + sr) * ld + k0 + sc]);
+            *(d2 *)&tile[sr * STR + sc] = __builtin_nontemporal_load(src);
         }
-        tile[sr * STR + sc] = v;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int e = t + 256 * i;
+            const int sr = e >> 6, sc = e & 63;
+            double v = 0.0;
+            if (!COLS) {   // source row = line, source column = k
+                if (l0 + sr < lines && k0 + sc < k) v = X[(l0 + sr) * ld + k0 + sc];
+            } else {       // source row = k, source column = line
+                if (k0 + sr < k && l0 + sc < lines) v = X[(k0 + sr) * ld + l0 + sc];
+            }
+            tile[sr * STR + sc] = v;
+        }
     }
     __syncthreads();
     const int lane = t & 63;
