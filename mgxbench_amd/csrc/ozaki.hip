@@ -41,6 +41,7 @@
 #include "modular.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace {
 
@@ -837,7 +838,19 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINB) void bf16_product_kernel(Prod
 // quad, lane) of the product kernel's layout, so each wave reads one
 // contiguous 1 KiB run per modulus; for each of its 16 outputs it gathers the
 // n residues, recovers C' exactly, rounds once and scales.
-template <int TB, int WGM, int WGN>
+// NM > 0 fixes the modulus count at compile time (the table words then fold
+// into immediates and the unused moduli drop out); NM = 0 reads it from nmod.
+template <int NM>
+__device__ __forceinline__ double crt_value(const uint32_t (&res)[MAX_MODULI], int nmod) {
+    if constexpr (NM > 0) {
+        constexpr mgx::modular::CrtTables T = mgx::modular::make_crt_tables();
+        return mgx::modular::reconstruct(T, res, NM);
+    } else {
+        return mgx::modular::reconstruct(CRT_TABLES, res, nmod);
+    }
+}
+
+template <int TB, int WGM, int WGN, int NM>
 __global__ __launch_bounds__(256) void reconstruct_kernel(const uint8_t *__restrict__ R,
                                                           const int32_t *__restrict__ rexp,
                                                           const int32_t *__restrict__ cexp,
@@ -845,6 +858,7 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(const uint8_t *__restr
                                                           int64_t N, int64_t ldc, int nmod,
                                                           int beta, int accumulate) {
     using S = TileShape<TB, WGM, WGN>;
+    const int nm = NM > 0 ? NM : nmod;
     const int tiles_m = (int)((M + TB - 1) / TB);
     const int tiles_n = (int)((N + TB - 1) / TB);
     const int64_t tiles = (int64_t)tiles_m * tiles_n;
@@ -863,7 +877,7 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(const uint8_t *__restr
         v4i runs[MAX_MODULI];
 #pragma unroll
         for (int l = 0; l < MAX_MODULI; ++l)
-            if (l < nmod)
+            if (l < nm)
                 runs[l] = *(const v4i *)(R + residue_run(l, tiles, tile, S::NW, wave, S::QPL, q) +
                                          lane * 16);
 #pragma unroll
@@ -878,11 +892,11 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(const uint8_t *__restr
                     uint32_t res[MAX_MODULI];
 #pragma unroll
                     for (int l = 0; l < MAX_MODULI; ++l)
-                        res[l] = l < nmod ? ((uint32_t)runs[l][w] >> (8 * r)) & 0xffu : 0u;
+                        res[l] = l < nm ? ((uint32_t)runs[l][w] >> (8 * r)) & 0xffu : 0u;
                     const int er = rexp[row];
                     const double x = (er == EXP_BAD || ec == EXP_BAD)
                                          ? __longlong_as_double(0x7ff8000000000000ll)
-                                         : ldexp(mgx::modular::reconstruct(CRT_TABLES, res, nmod),
+                                         : ldexp(crt_value<NM>(res, nm),
                                                  er + ec + shift);
                     double *c = C + row * ldc + col;
                     *c = accumulate ? *c + x : x;
@@ -1133,14 +1147,23 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
         const int64_t total = tiles * threads;
         const int64_t blocks = (total + 255) / 256;
         MGX_ARG_CHECK(blocks <= 0x7fffffffll, "matrix too large for the reconstruction");
-        if (tile == MGX_TILE_LARGE)
-            hipLaunchKernelGGL((reconstruct_kernel<256, 2, 4>), dim3((unsigned)blocks), dim3(256),
-                               0, st, Rres, rexp, cexp, C, m, n, ldc, slices, beta,
-                               accumulate ? 1 : 0);
-        else
-            hipLaunchKernelGGL((reconstruct_kernel<128, 2, 2>), dim3((unsigned)blocks), dim3(256),
-                               0, st, Rres, rexp, cexp, C, m, n, ldc, slices, beta,
-                               accumulate ? 1 : 0);
+        // the common modulus counts get compile-time specialisations
+        auto launch = [&](auto tb, auto nm) {
+            constexpr int TBv = decltype(tb)::value, NMv = decltype(nm)::value;
+            hipLaunchKernelGGL((reconstruct_kernel<TBv, 2, TBv == 256 ? 4 : 2, NMv>),
+                               dim3((unsigned)blocks), dim3(256), 0, st, Rres, rexp, cexp, C, m,
+                               n, ldc, slices, beta, accumulate ? 1 : 0);
+        };
+        using L = std::integral_constant<int, 256>;
+        using Sm = std::integral_constant<int, 128>;
+        auto by_count = [&](auto tb) {
+            if (slices == 15) launch(tb, std::integral_constant<int, 15>{});
+            else if (slices == 16) launch(tb, std::integral_constant<int, 16>{});
+            else if (slices == 14) launch(tb, std::integral_constant<int, 14>{});
+            else launch(tb, std::integral_constant<int, 0>{});
+        };
+        if (tile == MGX_TILE_LARGE) by_count(L{});
+        else by_count(Sm{});
         if ((rc = mgx::launch_status("reconstruct_kernel")) != MGX_OK) return rc;
     }
     return MGX_OK;
