@@ -22,6 +22,10 @@ Subcommands (python -m mgxbench_amd.autotune ...):
   evaluate  held-out MAPE and regret of the MLP vs the reference-style
             per-candidate power law and vs the fixed default configuration
   select    rank candidates for one shape (optionally time the top k)
+  share     the same for one rank's share of a 1D / 2D run
+
+The reference's run-level AUTO path (calibrated library / grid / BK choice)
+is mgxbench_amd/predictive.py.
 """
 from __future__ import annotations
 
@@ -388,12 +392,13 @@ def default_model_path() -> Path:
     return Path(__file__).resolve().parent / "models" / "mlp_cost_model.json"
 
 
-def auto(args) -> int:
-    """--libraries AUTO: pick (form, slices, tile, panel) for the run's rank share."""
+def share(args) -> int:
+    """Pick (form, slices, tile, panel) for one rank's local product of a
+    1D / 2D run (the `share` subcommand)."""
     from .grid import resolve_grid
     path = Path(args.auto_model) if args.auto_model else default_model_path()
     if not path.is_file():
-        print(f"AUTO: no cost model at {path}; train one with `python -m mgxbench_amd.autotune "
+        print(f"no cost model at {path}; train one with `python -m mgxbench_amd.autotune "
               "train`", file=sys.stderr)
         return 2
     model = MLPModel.from_json(json.loads(path.read_text()))
@@ -407,7 +412,7 @@ def auto(args) -> int:
     Path(args.output).mkdir(parents=True, exist_ok=True)
     dest = Path(args.output) / "autotuning_prediction.json"
     dest.write_text(json.dumps(out, indent=2))
-    print(f"AUTO choice for share m={m} n={n} k={k}: {choice['key']} "
+    print(f"choice for share m={m} n={n} k={k}: {choice['key']} "
           f"(predicted {choice['predicted_ms']:.2f} ms)")
     print(f"prediction written to {dest}")
     if args.auto_feedback and timer is not None:
@@ -449,7 +454,20 @@ def main(argv=None) -> int:
     p.add_argument("--n", type=int, required=True)
     p.add_argument("--k", type=int, required=True)
     p.add_argument("--validate", action="store_true")
+    p = sub.add_parser("share", help="pick the local-product configuration of a rank share")
+    p.add_argument("--decomposition", choices=("1d", "2d"), required=True)
+    p.add_argument("--size", type=int, required=True)
+    p.add_argument("--ranks", type=int, required=True)
+    p.add_argument("--grid-rows", type=int, default=None)
+    p.add_argument("--grid-cols", type=int, default=None)
+    p.add_argument("--auto-model", default=None, help="cost model JSON (default: the committed one)")
+    p.add_argument("--auto-validate", action="store_true", help="time the top candidates")
+    p.add_argument("--auto-feedback", action="store_true",
+                   help="append the timed candidates to the model's dataset")
+    p.add_argument("--output", default="results/autotune")
     a = ap.parse_args(argv)
+    if a.cmd == "share":
+        return share(a)
     if a.cmd == "measure":
         sizes = [s for s in SIZES if s <= a.max_size]
         shapes = rank_shares(sizes) + random_shapes(a.random_shapes, sizes=sizes)

@@ -289,7 +289,7 @@ def main(argv: list[str] | None = None) -> int:
     args = ap.parse_args(argv)
     grid = validate_args(ap, args)
     if args.libraries == ["AUTO"]:
-        from .autotune import auto
+        from .predictive import auto
         return auto(args)
     for lib in args.libraries:
         if len(lib) != 3 or any(c not in "MYCXNSW" for c in lib):
