@@ -14,7 +14,7 @@ and moduli counts whose scaling exponent at the panel depth is at least
 MIN_BETA (mgx_split_mod_beta).  One MLP (3 layers, width 64,
 FP32, trained on the GPU by libmgx's own kernels) predicts log runtime from
 log-shape features for every candidate at once; exhaustive timing of its
-top-k candidates stays the fallback and the judge.
+top-k candidates stays the fallback and the arbiter.
 
 Subcommands (python -m mgxbench_amd.autotune ...):
   measure   time candidates on the GPU -> dataset JSON (checkpointed per shape)
