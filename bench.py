@@ -59,6 +59,8 @@ def parse_args():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target CPU time of the CPU-baseline row slab")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--bf16", action="store_true",
+                    help="also time the BF16 digit-slice form (same slices as --slices)")
     ap.add_argument("--no-summa", action="store_true",
                     help="skip the host-streamed SUMMA panel report")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -176,6 +178,8 @@ def main() -> int:
     mgx.fill(A, m, k, k, 1, seed=1)
     mgx.fill(B, k, n, n, 1, seed=2)
     forms = {"mod": (mgx.FORM_INT8_MOD, args.moduli), "slices": (mgx.FORM_INT8, args.slices)}
+    if args.bf16:
+        forms["bf16"] = (mgx.FORM_BF16, args.slices)
     ws_bytes = max(mgx.split_workspace_bytes(f, p, m, n, k) for f, p in forms.values())
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
@@ -248,10 +252,12 @@ def main() -> int:
         acc = mgx.error_report(A, B, C, m, n, k, k, n, n, rows)
         acc.pop("rows_checked", None)
         products = planes if name == "mod" else planes * (planes + 1) // 2
+        keep = C.clone() if args.bf16 and name in ("slices", "bf16") else None
         return {"ms": ms, "phases": phases, "diff": diff, "acc": acc, "products": products,
+                "result": keep,
                 "planes": planes}
 
-    res = {name: measure(name) for name in ("mod", "slices")}
+    res = {name: measure(name) for name in forms}
     head = res["mod"]
     log(f"normwise error modular {head['acc']['normwise']:.3e} slices "
         f"{res['slices']['acc']['normwise']:.3e} plain {acc_plain['normwise']:.3e}")
@@ -304,6 +310,18 @@ def main() -> int:
             "phase_ms": {p: round(v, 4) for p, v in sl["phases"].items() if p != "reconstruction"},
             "no_worse_than_plain": sl["acc"]["normwise"] <= acc_plain["normwise"],
         },
+        **({"split_bf16": {
+            "form": (f"bf16 digit-slice split, s={res['bf16']['planes']} (the INT8 form's "
+                     "digits on the bf16 matrix cores; same result)"),
+            "tflops": round(F / (res["bf16"]["ms"] * 1e-3) / 1e12, 3),
+            "ms_per_step": round(res["bf16"]["ms"], 4),
+            "speedup_vs_plain": round(ms_plain / res["bf16"]["ms"], 4),
+            "bf16_products_tflops": round(res["bf16"]["products"] * F /
+                                          (res["bf16"]["phases"]["int8_products"] * 1e-3) / 1e12,
+                                          2),
+            "bitwise_equal_to_int8_form": bool(torch.equal(res["bf16"]["result"],
+                                                           res["slices"]["result"])),
+        }} if "bf16" in res else {}),
         "roofline": {
             "kernel": "int8_product_kernel (per-modulus INT8 residue products)",
             "bound": "mfma",
