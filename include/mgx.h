@@ -128,8 +128,9 @@ int mgx_validate_constant(const double *dC, int64_t rows, int64_t cols,
  * Its events: [2] before the residue products, [3] after them (the
  * reconstruction into C follows [3]).
  * MGX_FORM_BF16: `slices` 1..9 as for MGX_FORM_INT8, same digits, group
- * sums, chunks and folds, so the same result bit for bit; always runs the
- * MGX_TILE_SMALL block tile (it keeps FP32 and INT32 accumulators). */
+ * sums, chunks and folds, so the same result bit for bit; always runs its
+ * own 256 x 128 block tile (it keeps FP32 and INT32 accumulators) and
+ * ignores `tile`. */
 size_t mgx_split_workspace_bytes(int form, int slices,
                                  int64_t m, int64_t n, int64_t k);
 int mgx_split_gemm(int form, int slices,

@@ -344,10 +344,11 @@ __global__ __launch_bounds__(256) void operand_tile_kernel(const double *__restr
 
 // ---- INT8 product kernel ------------------------------------------------------
 //
-// One workgroup owns a TB x TB block of C.  TB = 256: 8 wave64s (2 x 4), two
-// per SIMD, each holding a 128 x 64 INT32 accumulator tile; TB = 128: 4 waves
-// (2 x 2) of 64 x 64, two workgroups per CU.  Two waves per SIMD let one
-// wave's LDS reads, barrier wait and load issue overlap the other's MFMAs.
+// One workgroup owns a block of C.  256 x 256: 8 wave64s (2 x 4), two per
+// SIMD, each holding a 128 x 64 INT32 accumulator tile; 128 x 128: 4 waves
+// (2 x 2) of 64 x 64, two workgroups per CU; 256 x 128 (bf16 form): 8 waves
+// (4 x 2) of 64 x 64.  Two waves per SIMD let one wave's LDS reads, barrier
+// wait and load issue overlap the other's MFMAs.
 //
 // Operand staging: a ring of NSTAGE LDS slots, each holding the A and B tiles
 // of one 64-byte k stage, is filled by direct global->LDS loads
@@ -558,25 +559,36 @@ __device__ __forceinline__ int64_t residue_run(int l, int64_t tiles, int64_t til
     return ((((int64_t)l * tiles + tile) * nw + wave) * qpl + q) * 1024;
 }
 
-template <int TB, int WGM, int WGN>
+template <int TBM_, int TBN_, int WGM, int WGN>
 struct TileShape {
+    static constexpr int TBM = TBM_;               // C block rows
+    static constexpr int TBN = TBN_;               // C block columns
     static constexpr int NW = WGM * WGN;           // waves per workgroup
     static constexpr int THREADS = 64 * NW;
-    static constexpr int WTM = TB / WGM;           // wave tile rows
-    static constexpr int WTN = TB / WGN;           // wave tile columns
+    static constexpr int WTM = TBM / WGM;          // wave tile rows
+    static constexpr int WTN = TBN / WGN;          // wave tile columns
     static constexpr int FI = WTM / 16;            // 16x16 fragments along rows
     static constexpr int FJ = WTN / 16;            // ... along columns
     static constexpr int QPL = FI * FJ / 4;        // 16-byte residue runs per lane
-    static constexpr int TILE = TB * KT;           // bytes of one operand tile per stage
-    static constexpr int SLOT = 2 * TILE;          // A tile then B tile
-    static constexpr int LPW = TILE / 1024 / NW;   // 1 KiB loads per wave per operand per stage
+    static constexpr int TILE_A = TBM * KT;        // bytes of the A tile per stage
+    static constexpr int TILE_B = TBN * KT;        // bytes of the B tile per stage
+    static constexpr int SLOT = TILE_A + TILE_B;   // A tile then B tile
+    static constexpr int LPW_A = TILE_A / 1024 / NW;   // 1 KiB loads per wave per stage, A
+    static constexpr int LPW_B = TILE_B / 1024 / NW;   // ... B
     static constexpr int LDS_BYTES = NSTAGE * SLOT;
-    static_assert(LPW >= 1 && LPW * NW * 1024 == TILE, "loads must cover the tile");
+    static_assert(LPW_A >= 1 && LPW_A * NW * 1024 == TILE_A, "loads must cover the A tile");
+    static_assert(LPW_B >= 1 && LPW_B * NW * 1024 == TILE_B, "loads must cover the B tile");
     static_assert(LDS_BYTES <= 160 * 1024, "LDS budget");
     static_assert(FI % 2 == 0 && (FI * FJ) % 4 == 0, "fragment grouping");
 };
-using ShapeLarge = TileShape<256, 2, 4>;
-using ShapeSmall = TileShape<128, 2, 2>;
+using ShapeLarge = TileShape<256, 256, 2, 4>;
+using ShapeSmall = TileShape<128, 128, 2, 2>;
+// bf16 form: 8 waves of 64 x 64 (its FP32 and INT32 accumulator sets fit two
+// waves per SIMD) in one workgroup per CU.  Two co-resident 128-tile
+// workgroups per CU drift apart (the older one's waves win the matrix-core
+// issue slots), which spoils the L2 sharing between the workgroups of a
+// raster group; one workgroup per CU keeps its waves in step at the barriers.
+using ShapeWide = TileShape<256, 128, 4, 2>;
 
 struct ProductArgs {
     const int8_t *A, *B;          // operand planes, k-tile blocked
@@ -598,18 +610,19 @@ struct ProductArgs {
 // the result is bit-identical to it.
 constexpr int BF_FLUSH_STEPS = 16;
 
-template <int TB, int WGM, int WGN, bool MOD, bool BF>
+template <int TBM, int TBN, int WGM, int WGN, bool MOD, bool BF>
 __device__ __forceinline__ void product_body(const ProductArgs &a) {
     static_assert(!(MOD && BF), "bf16 planes carry digit slices only");
-    using S = TileShape<TB, WGM, WGN>;
-    constexpr int FI = S::FI, FJ = S::FJ, TILE = S::TILE, SLOT = S::SLOT, LPW = S::LPW;
+    using S = TileShape<TBM, TBN, WGM, WGN>;
+    constexpr int FI = S::FI, FJ = S::FJ, TILE_A = S::TILE_A, SLOT = S::SLOT;
+    constexpr int LPW_A = S::LPW_A, LPW_B = S::LPW_B;
     // operand ring
     __shared__ __attribute__((aligned(1024))) int8_t lds[S::LDS_BYTES];
 
     const int64_t M = a.M, N = a.N;
     const int s = a.s;
-    const int tiles_m = (int)((M + TB - 1) / TB);
-    const int tiles_n = (int)((N + TB - 1) / TB);
+    const int tiles_m = (int)((M + TBM - 1) / TBM);
+    const int tiles_n = (int)((N + TBN - 1) / TBN);
     int tm, tn;
     {
         // XCD-aware remap: consecutive workgroups go to different XCDs, so
@@ -625,9 +638,9 @@ __device__ __forceinline__ void product_body(const ProductArgs &a) {
         tm = first_m + (b % per_group) % gsize;
         tn = (b % per_group) / gsize;
     }
-    const int64_t m0 = (int64_t)tm * TB;
-    const int64_t n0 = (int64_t)tn * TB;
-    const bool full_tile = m0 + TB <= M && n0 + TB <= N;
+    const int64_t m0 = (int64_t)tm * TBM;
+    const int64_t n0 = (int64_t)tn * TBN;
+    const bool full_tile = m0 + TBM <= M && n0 + TBN <= N;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -636,30 +649,33 @@ __device__ __forceinline__ void product_body(const ProductArgs &a) {
     const int lk = lane >> 4;
     const int lc = lane & 15;
 
-    // Direct-load assignment: load j of this wave fills tile rows
-    // 16*(wave*LPW + j) .. +16; lane l reads row +l/4, 16-byte chunk
-    // (l%4) ^ swz(l/4), landing at position l%4 of that row in LDS.
-    // Consecutive loads are 1 KiB apart in HBM and in LDS.
-    const int lane_off = (16 * wave * LPW + (lane >> 2)) * KT + (((lane & 3) ^ swz(lane >> 2)) << 4);
+    // Direct-load assignment: load j of this wave fills rows
+    // 16*(wave*LPW + j) .. +16 of the A (B) tile, LPW = LPW_A (LPW_B); lane l
+    // reads row +l/4, 16-byte chunk (l%4) ^ swz(l/4), landing at position
+    // l%4 of that row in LDS.  Consecutive loads are 1 KiB apart in HBM and
+    // in LDS.
+    const int lane_chunk = (lane >> 2) * KT + (((lane & 3) ^ swz(lane >> 2)) << 4);
 
     v4i acc[FI][FJ];
     const v4i vzero = {0, 0, 0, 0};
     v4f accf[BF ? FI : 1][BF ? FJ : 1];   // bf16 form: FP32 partial sums
-    constexpr int LPS = 2 * LPW;   // load instructions per wave per stage
+    constexpr int LPS = LPW_A + LPW_B;   // load instructions per wave per stage
 
     const int ktiles = (int)a.KTC;
     LoadCursor<MOD, BF> cur;
     cur.start(s, ktiles, a.A + m0 * KT, a.B + n0 * KT, a.Mp * KT, a.Np * KT, a.KTC * a.Mp * KT,
               a.KTC * a.Np * KT);
-    const uint32_t voff = (uint32_t)lane_off;
-    int8_t *const lds_w = lds + wave * LPW * 1024;   // this wave's load runs
+    const uint32_t voff_a = (uint32_t)(16 * wave * LPW_A * KT + lane_chunk);
+    const uint32_t voff_b = (uint32_t)(16 * wave * LPW_B * KT + lane_chunk);
+    int8_t *const lds_wa = lds + wave * LPW_A * 1024;            // this wave's A load runs
+    int8_t *const lds_wb = lds + TILE_A + wave * LPW_B * 1024;   // ... and B load runs
     auto issue = [&](int slot) {
-        int8_t *la = lds_w + slot * SLOT;
 #pragma unroll
-        for (int j = 0; j < LPW; ++j) {
-            dma16(cur.ga + voff + j * 1024, la + j * 1024);
-            dma16(cur.gb + voff + j * 1024, la + TILE + j * 1024);
-        }
+        for (int j = 0; j < LPW_A; ++j)
+            dma16(cur.ga + voff_a + j * 1024, lds_wa + slot * SLOT + j * 1024);
+#pragma unroll
+        for (int j = 0; j < LPW_B; ++j)
+            dma16(cur.gb + voff_b + j * 1024, lds_wb + slot * SLOT + j * 1024);
         cur.advance();
     };
     // prologue: stages 0 .. NSTAGE-2.  Every step then issues exactly one
@@ -690,7 +706,7 @@ __device__ __forceinline__ void product_body(const ProductArgs &a) {
     };
     auto read_hi = [&](v4i (&fa)[FH], v4i (&fb)[FJ]) {
         const int8_t *la = lds + pslot * SLOT;
-        const int8_t *lb = la + TILE;
+        const int8_t *lb = la + TILE_A;
 #pragma unroll
         for (int i = 0; i < FH; ++i) {
             const int r = wm * S::WTM + (FH + i) * 16 + lc;
@@ -826,12 +842,12 @@ __device__ __forceinline__ void product_body(const ProductArgs &a) {
 
 template <int TB, int WGM, int WGN, bool MOD, int MINB>
 __global__ __launch_bounds__(64 * WGM * WGN, MINB) void int8_product_kernel(ProductArgs a) {
-    product_body<TB, WGM, WGN, MOD, false>(a);
+    product_body<TB, TB, WGM, WGN, MOD, false>(a);
 }
 
-template <int TB, int WGM, int WGN, int MINB>
+template <int TBM, int TBN, int WGM, int WGN, int MINB>
 __global__ __launch_bounds__(64 * WGM * WGN, MINB) void bf16_product_kernel(ProductArgs a) {
-    product_body<TB, WGM, WGN, false, true>(a);
+    product_body<TBM, TBN, WGM, WGN, false, true>(a);
 }
 
 // Scheme II reconstruction: one thread per (block tile, wave, fragment
@@ -857,7 +873,7 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(const uint8_t *__restr
                                                           double *__restrict__ C, int64_t M,
                                                           int64_t N, int64_t ldc, int nmod,
                                                           int beta, int accumulate) {
-    using S = TileShape<TB, WGM, WGN>;
+    using S = TileShape<TB, TB, WGM, WGN>;
     const int nm = NM > 0 ? NM : nmod;
     const int tiles_m = (int)((M + TB - 1) / TB);
     const int tiles_n = (int)((N + TB - 1) / TB);
@@ -1004,10 +1020,10 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
                                                        ? "modulus count out of range"
                                                        : "slice count out of range");
     const bool mod = form == MGX_FORM_INT8_MOD;
-    // the bf16 form keeps FP32 and INT32 accumulators: the 128 tile only
+    // the bf16 form runs its own 256 x 128 block tile whatever `tile` asks
     const bool bf = form == MGX_FORM_BF16;
-    if (bf) tile = MGX_TILE_SMALL;
-    const int64_t TBsz = tile == MGX_TILE_LARGE ? 256 : 128;
+    const int64_t TBm = bf ? ShapeWide::TBM : tile == MGX_TILE_LARGE ? 256 : 128;
+    const int64_t TBn = bf ? ShapeWide::TBN : TBm;
     MGX_ARG_CHECK(m >= 0 && n >= 0 && k >= 0, "negative matrix dimension");
     if (m == 0 || n == 0) return MGX_OK;
     MGX_ARG_CHECK(C != nullptr && ldc >= n, "C is NULL or ldc < n");
@@ -1024,7 +1040,7 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     if (mod && beta < 0)
         return mgx::set_error(MGX_ERR_UNSUPPORTED, "too few moduli for this k");
     const SplitLayout L = split_layout(form, slices, m, n, k);
-    const int64_t tiles = ((m + TBsz - 1) / TBsz) * ((n + TBsz - 1) / TBsz);
+    const int64_t tiles = ((m + TBm - 1) / TBm) * ((n + TBn - 1) / TBn);
     MGX_ARG_CHECK(tiles <= 0x7fffffffll, "matrix too large for one launch");
 
     char *ws = (char *)workspace;
@@ -1122,8 +1138,8 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     pa.s = slices;
     pa.accumulate = accumulate ? 1 : 0;
     if (bf) {
-        hipLaunchKernelGGL((bf16_product_kernel<128, 2, 2, 2>), dim3((unsigned)tiles),
-                           dim3(ShapeSmall::THREADS), 0, st, pa);
+        hipLaunchKernelGGL((bf16_product_kernel<256, 128, 4, 2, 1>), dim3((unsigned)tiles),
+                           dim3(ShapeWide::THREADS), 0, st, pa);
     } else if (tile == MGX_TILE_LARGE) {
         if (mod)
             hipLaunchKernelGGL((int8_product_kernel<256, 2, 4, true, 1>), dim3((unsigned)tiles),

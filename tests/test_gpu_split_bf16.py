@@ -45,7 +45,7 @@ def test_bit_exact_vs_int8_model_every_slice_count(mgx, orc, s):
                                      (64, 300, 1000, 7), (300, 260, 2000, 2)])
 @pytest.mark.parametrize("tile", [1, 2])
 def test_shapes_and_tile_argument(mgx, orc, m, n, k, s, tile):
-    # the form always runs the 128 tile; either request gives the same result
+    # the form always runs its own 256 x 128 tile; either request gives the same result
     announce(f"split bf16 shape {m}x{n}x{k} s={s} tile={tile}")
     A = orc.fill(m, k, 1, seed=m + k)
     B = orc.fill(k, n, 1, seed=n + k)
