@@ -89,153 +89,110 @@ __host__ __device__ inline int residue_int8_narrow(double x, int l) {
     return r;
 }
 
-// Correctly rounded (to nearest, ties to even) conversion of the 128-bit
-// magnitude hi:lo to double.
-__host__ __device__ inline double u128_to_double_rn(uint64_t hi, uint64_t lo) {
-    if (hi == 0 && lo == 0) return 0.0;
-    uint64_t top, rest;
-    int e;
-    if (hi != 0) {
-        const int s = __builtin_clzll(hi);
-        top = s ? (hi << s) | (lo >> (64 - s)) : hi;
-        rest = s ? lo << s : lo;
-        e = 75 - s;   // value = (top 2^64 + rest) 2^-s,  top = mant 2^11 + round bits
-    } else {
-        const int s = __builtin_clzll(lo);
-        top = lo << s;
-        rest = 0;
-        e = 11 - s;
-    }
-    uint64_t mant = top >> 11;
-    const uint32_t rb = (uint32_t)(top & 0x7ffu);
-    if (rb > 0x400u || (rb == 0x400u && (rest != 0 || (mant & 1u)))) ++mant;   // mant <= 2^53
-    return ldexp((double)mant, e);
-}
-
-// Direct Chinese remaindering tables for n = 2..16 moduli:
+// Direct Chinese remaindering tables for n = 2..16 moduli, in FP64:
 //   x = (sum_l r_l W_l) mod P_n,   W_l = (P_n / m_l) ((P_n / m_l)^-1 mod m_l) < P_n,
-// with W and P_n in eight 16-bit limbs and 1/P_n as a double.
+// with W and P_n in four 40-bit limbs held exactly as doubles, fl(1/P_n),
+// and a threshold just below P_n / 2 for the exact range check.
 struct CrtTables {
-    uint32_t w[MAX_MODULI + 1][MAX_MODULI][8];
-    uint32_t p[MAX_MODULI + 1][8];
+    double w[MAX_MODULI + 1][MAX_MODULI][4];
+    double p[MAX_MODULI + 1][4];
     double inv_p[MAX_MODULI + 1];
+    double half_p_lo[MAX_MODULI + 1];
 };
+
+constexpr int LIMB_BITS = 40;
 
 constexpr CrtTables make_crt_tables() {
     CrtTables T{};
+    constexpr unsigned __int128 MASK = ((unsigned __int128)1 << LIMB_BITS) - 1;
     for (int n = MIN_MODULI; n <= MAX_MODULI; ++n) {
         unsigned __int128 P = 1;
         for (int l = 0; l < n; ++l) P *= MODULI[l];
-        for (int w = 0; w < 8; ++w) T.p[n][w] = (uint32_t)(P >> (16 * w)) & 0xffffu;
+        for (int j = 0; j < 4; ++j) T.p[n][j] = (double)(uint64_t)((P >> (LIMB_BITS * j)) & MASK);
         // 1/P rounded twice (conversion of P, division); only used for an estimate
         T.inv_p[n] = 1.0 / (double)P;
+        // P/2 (1 - 2^-36), rounded: below P/2 (1 - 2^-37)
+        T.half_p_lo[n] = (double)(P >> 1) * (1.0 - 0x1p-36);
         for (int l = 0; l < n; ++l) {
             const unsigned __int128 Ml = P / MODULI[l];
             const uint32_t y = inverse_mod((uint32_t)(Ml % MODULI[l]), MODULI[l]);
             const unsigned __int128 W = Ml * y;
-            for (int w = 0; w < 8; ++w) T.w[n][l][w] = (uint32_t)(W >> (16 * w)) & 0xffffu;
+            for (int j = 0; j < 4; ++j)
+                T.w[n][l][j] = (double)(uint64_t)((W >> (LIMB_BITS * j)) & MASK);
         }
     }
     return T;
 }
 
-// Exact integer x with x = r_l (mod m_l), |x| < P/2, from residues
+// Carries the integer-valued limbs x[0..2] into [0, 2^40) (floor division by
+// 2^40); the value sum x_j 2^(40 j) is unchanged and x[3] keeps its sign.
+// Exact while every |x_j| stays below 2^53 - 2^14.
+__host__ __device__ inline void normalize40(double (&x)[4]) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const double c = floor(x[j] * 0x1p-40);
+        x[j] = fma(-c, 0x1p40, x[j]);
+        x[j + 1] += c;
+    }
+}
+
+// Exact integer x with x = r_l (mod m_l), -P/2 < x <= P/2, from residues
 // r_l in [0, m_l), l < n, rounded to double (T = make_crt_tables()).
-//   S = sum_l r_l W_l       (16-bit limbs, each limb sum < 16 * 2^24)
-//   q = floor(S / P) from a double estimate, corrected by one either way
-//   x = S - q P in [0, P);  C' = x, or x - P when x > P / 2.
+//   S_j = sum_l r_l W_lj          exact: every partial sum < 16 * 255 * 2^40 < 2^52
+//   q = rint(S / P) from a double estimate (error < 2^-40), so that
+//   x = S - q P (limb by limb, exact) is the symmetric representative unless
+//   |x| lies within 2^-39 P of P/2; that case is recognised from |x| >= the
+//   threshold and settled exactly.  |x| then goes to double with one rounding.
 __host__ __device__ inline double reconstruct(const CrtTables &T, const uint32_t (&r)[MAX_MODULI],
                                               int n) {
-    const uint32_t(&W)[MAX_MODULI][8] = T.w[n];
-    const uint32_t(&P)[8] = T.p[n];
-    uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const double(&W)[MAX_MODULI][4] = T.w[n];
+    const double(&P)[4] = T.p[n];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int l = 0; l < MAX_MODULI; ++l) {
-        const uint32_t rl = l < n ? (r[l] & 0xffu) : 0u;
+        const double rl = l < n ? (double)(r[l] & 0xffu) : 0.0;
 #pragma unroll
-        for (int w = 0; w < 8; ++w) acc[w] += rl * (W[l][w] & 0xffffu);
+        for (int j = 0; j < 4; ++j) s[j] = fma(rl, W[l][j], s[j]);
     }
-    // normalise to 16-bit limbs s[0..8] (S < 16 * 256 * P < 2^139)
-    uint32_t sl[9];
-    uint32_t carry = 0;
+    const double sd = fma(fma(fma(s[3], 0x1p40, s[2]), 0x1p40, s[1]), 0x1p40, s[0]);
+    const double q = rint(sd * T.inv_p[n]);   // 0 <= q <= 4080
+    double x[4];
 #pragma unroll
-    for (int w = 0; w < 8; ++w) {
-        const uint32_t t = acc[w] + carry;
-        sl[w] = t & 0xffffu;
-        carry = t >> 16;
-    }
-    sl[8] = carry;
-    double sd = 0.0;
+    for (int j = 0; j < 4; ++j) x[j] = fma(-q, P[j], s[j]);   // |x_j| < 2^52
+    normalize40(x);
+    // x < 0 iff its top limb is; negate limb by limb and carry again: |x|
+    double sg = x[3] < 0.0 ? -1.0 : 1.0;
 #pragma unroll
-    for (int w = 8; w >= 0; --w) sd = sd * 65536.0 + (double)sl[w];
-    int32_t q = (int32_t)floor(sd * T.inv_p[n]);   // 0 <= q < 4096, off by at most one
-    // x = S - q P (signed 16-bit limb arithmetic with borrow)
-    int32_t xl[9];
-    int32_t borrow = 0;
+    for (int j = 0; j < 4; ++j) x[j] *= sg;
+    normalize40(x);
+    const double mag = fma(fma(fma(x[3], 0x1p40, x[2]), 0x1p40, x[1]), 0x1p40, x[0]);
+    if (mag >= T.half_p_lo[n]) {
+        // t = 2|x| - P, exactly; |x| > P/2 (or x = -P/2) takes the other representative
+        double t[4];
 #pragma unroll
-    for (int w = 0; w < 9; ++w) {
-        const int32_t pw = w < 8 ? (int32_t)(P[w] & 0xffffu) : 0;
-        int32_t t = (int32_t)sl[w] - q * pw - borrow;   // |q * pw| < 2^28
-        borrow = 0;
-        // floor division of t by 2^16 (arithmetic shift) -> next borrow
-        borrow = -(t >> 16);
-        xl[w] = t & 0xffff;
-    }
-    // the value is sum xl[w] 2^(16w) - borrow 2^144; borrow != 0 means x < 0
-    auto add_p = [&](int sign) {
-        int32_t c = 0;
+        for (int j = 0; j < 4; ++j) t[j] = fma(2.0, x[j], -P[j]);
+        normalize40(t);
+        const bool zero = t[0] == 0.0 && t[1] == 0.0 && t[2] == 0.0 && t[3] == 0.0;
+        if ((t[3] >= 0.0 && !zero) || (zero && sg < 0.0)) {
 #pragma unroll
-        for (int w = 0; w < 9; ++w) {
-            const int32_t pw = w < 8 ? (int32_t)(P[w] & 0xffffu) : 0;
-            const int32_t t = xl[w] + sign * pw + c;
-            c = t >> 16;
-            xl[w] = t & 0xffff;
-        }
-        return c;
-    };
-    if (borrow != 0) borrow -= add_p(1);   // q was one too large
-    // x >= P ?  compare limbs from the top
-    {
-        bool ge = true, decided = false;
-#pragma unroll
-        for (int w = 8; w >= 0; --w) {
-            const uint32_t pw = w < 8 ? (P[w] & 0xffffu) : 0u;
-            if (!decided && (uint32_t)xl[w] != pw) {
-                ge = (uint32_t)xl[w] > pw;
-                decided = true;
-            }
-        }
-        if (ge) add_p(-1);   // q was one too small
-    }
-    // symmetric range: x > P/2  <=>  2x > P
-    uint32_t x32[4] = {(uint32_t)xl[0] | ((uint32_t)xl[1] << 16),
-                       (uint32_t)xl[2] | ((uint32_t)xl[3] << 16),
-                       (uint32_t)xl[4] | ((uint32_t)xl[5] << 16),
-                       (uint32_t)xl[6] | ((uint32_t)xl[7] << 16)};
-    uint32_t p32[4] = {(P[0] & 0xffffu) | (P[1] << 16), (P[2] & 0xffffu) | (P[3] << 16),
-                       (P[4] & 0xffffu) | (P[5] << 16), (P[6] & 0xffffu) | (P[7] << 16)};
-    uint32_t y[4];
-    int64_t bo = 0;
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-        const int64_t t = (int64_t)p32[w] - (int64_t)x32[w] - bo;
-        y[w] = (uint32_t)t;
-        bo = t < 0 ? 1 : 0;
-    }
-    bool greater = false, decided = false;
-#pragma unroll
-    for (int w = 3; w >= 0; --w) {
-        if (!decided && x32[w] != y[w]) {
-            greater = x32[w] > y[w];
-            decided = true;
+            for (int j = 0; j < 4; ++j) x[j] = P[j] - x[j];
+            normalize40(x);
+            sg = -sg;
         }
     }
-    uint32_t mag[4];
-#pragma unroll
-    for (int w = 0; w < 4; ++w) mag[w] = greater ? y[w] : x32[w];
-    const double d = u128_to_double_rn(((uint64_t)mag[3] << 32) | mag[2],
-                                       ((uint64_t)mag[1] << 32) | mag[0]);
-    return greater ? -d : d;
+    // |x| = a + b + c with a = (x3 2^40 + x2) 2^80, b = x1 2^40, c = x0, each
+    // exact (x3 2^40 + x2 < 2^48, limbs in [0, 2^40)).  s + e = a + b exactly
+    // (fast two-sum: a = 0 or a > b).  e = 0: RN(s + c) is the one rounding.
+    // e != 0: a + b >= 2^93, so the result's half-ulp is a multiple of 2^40
+    // and any c in (0, 2^40) rounds like 2^39; e + 2^39 is exact (e is a
+    // multiple of 2^40 below 2^67), so RN(s + (e + sticky)) = RN(|x|).
+    const double a = fma(x[3], 0x1p40, x[2]) * 0x1p80;
+    const double b = x[1] * 0x1p40;
+    const double sum = a + b;
+    const double e = b - (sum - a);
+    const double c = e == 0.0 ? x[0] : e + (x[0] != 0.0 ? 0x1p39 : 0.0);
+    const double d = sum + c;
+    return sg < 0.0 ? -d : d;
 }
 
 }  // namespace modular

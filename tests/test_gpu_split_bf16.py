@@ -52,6 +52,16 @@ def test_shapes_and_tile_argument(mgx, orc, m, n, k, s, tile):
     assert _same(_bf16(mgx, A, B, s, tile=tile), orc.split_gemm_int8(A, B, s))
 
 
+def test_whole_tiles_and_raster_remap(mgx, orc):
+    # 1024 x 1024 is 4 x 8 whole 256 x 128 blocks: no edge checks in the fold,
+    # and a block count divisible by 8 takes the XCD-aware tile remap
+    m, n, k, s = 1024, 1024, 320, 3
+    announce(f"split bf16 whole tiles {m}x{n}x{k} s={s}")
+    A = orc.fill(m, k, 1, seed=61)
+    B = orc.fill(k, n, 1, seed=62)
+    assert _same(_bf16(mgx, A, B, s), orc.split_gemm_int8(A, B, s))
+
+
 @pytest.mark.parametrize("s", [3, 5, 8])
 def test_stress_scaling(mgx, orc, s):
     m, n, k = 96, 80, 500
