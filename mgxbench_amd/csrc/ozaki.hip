@@ -276,8 +276,11 @@ __global__ __launch_bounds__(256) void operand_tile_kernel(const double *__restr
                                                            int64_t KTC, int8_t *__restrict__ Xop) {
     constexpr int STR = 66;   // doubles per padded LDS row
     __shared__ double tile[64 * STR];
-    const int64_t kt = blockIdx.x;
-    const int64_t l0 = (int64_t)blockIdx.y * 64;
+    // Consecutive workgroups take consecutive 512-byte segments of the same
+    // 64 source rows (k tiles for A, line blocks for B), so DRAM pages are
+    // read in one pass.
+    const int64_t kt = COLS ? blockIdx.y : blockIdx.x;
+    const int64_t l0 = (int64_t)(COLS ? blockIdx.x : blockIdx.y) * 64;
     const int64_t k0 = kt * KT;
     const int t = threadIdx.x;
     // Stage the 64 x 64 source tile (source rows are lines for A, k for B).
@@ -1107,9 +1110,10 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
         if ((rc = mgx::launch_status("operand_tile_kernel (A)")) != MGX_OK) return rc;
     }
     {
-        const int64_t gy = L.Np / 64;
-        MGX_ARG_CHECK(gy <= 65535, "n too large for the column pass");
-        const dim3 grid((unsigned)(L.Kp / KT), (unsigned)gy);
+        // x: 64-column blocks of B, y: k tiles
+        MGX_ARG_CHECK(L.Kp / KT <= 65535 && L.Np / 64 <= 0x7fffffffll,
+                      "k or n too large for the column pass");
+        const dim3 grid((unsigned)(L.Np / 64), (unsigned)(L.Kp / KT));
         if (bf)
             hipLaunchKernelGGL((operand_tile_kernel<false, true, true>), grid, dim3(256), 0, st,
                                B, n, k, ldb, cexp, slices, 0, L.Np, L.KTC, Bop);
