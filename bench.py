@@ -158,6 +158,38 @@ def summa_report(mgx, torch, A, B, m, n, k, nmod) -> dict:
     return rep
 
 
+
+def mfma_ceiling(mgx, torch, stream, iters=12000, reps=3):
+    """INT8 matrix-core rate of an MFMA-only loop shaped like the product
+    kernel (mgx_mfma_i8_ceiling: one 8-wave workgroup per CU), with random
+    operand bytes re-read from LDS every round and with constant (all-ones)
+    register operands, in TOP/s (hipEvent timing on `stream`).  The random
+    rate is what the part sustains on data like the residue planes; the
+    dense peak assumes the full clock."""
+    blocks = torch.cuda.get_device_properties(0).multi_processor_count
+    out = torch.empty(blocks * 512, dtype=torch.int32, device="cuda")
+    lib = mgx._lib
+    ops = lib.lib().mgx_mfma_i8_ceiling_ops(blocks, iters)
+    rates = {}
+    for name, rnd in (("random_lds_operands_tops", 1), ("constant_operands_tops", 0)):
+        lib.call("mgx_mfma_i8_ceiling", out.data_ptr(), blocks, 2, rnd, stream.cuda_stream)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            lib.call("mgx_mfma_i8_ceiling", out.data_ptr(), blocks, iters, rnd, stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        rates[name] = round(ops / (ms * 1e-3) / 1e12, 2)
+        if not rnd and not bool(torch.all(out == 8192 * iters)):
+            raise RuntimeError("mfma ceiling probe: constant-operand result is wrong")
+    rates.update({"workgroups": blocks, "mfma_per_wave": iters * 32,
+                  "kind": ("v_mfma_i32_16x16x64_i8-only loop, 8 waves per CU, fragments "
+                           "re-read from LDS every round (no global loads, no barriers)")})
+    return rates
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -334,6 +366,11 @@ def main() -> int:
                               "x2.0 + WRITE_SIZE x1.0, calibrated on a 1 GiB copy)",
         },
     }
+    log("mfma ceiling probe")
+    ceiling = mfma_ceiling(mgx, torch, stream)
+    out["roofline"]["mfma_ceiling"] = ceiling
+    out["roofline"]["frac_of_random_operand_ceiling"] = round(
+        achieved / ceiling["random_lds_operands_tops"], 4)
     if not args.no_summa:
         out["summa_streamed"] = summa_report(mgx, torch, A, B, m, n, k, args.moduli)
     if not args.no_cpu_baseline:

@@ -277,6 +277,21 @@ int mgx_time_local_product(const mgx_tune_config *cfg,
                            void *workspace, size_t workspace_bytes,
                            int warmup, int reps, double *ms_out);
 
+/* ---- matrix-core ceiling probe ---------------------------------------------
+ * No counterpart in the reference: a measurement aid for the product
+ * kernel's roofline.  Launches `blocks` workgroups of 8 waves that issue
+ * only independent v_mfma_i32_16x16x64_i8 (8 x 4 accumulator fragments per
+ * wave, as in the product kernel), `iters` (even) rounds of 32 per wave.
+ * random_operands != 0: every round re-reads the wave's 12 fragments from a
+ * 64 KiB LDS buffer of pseudo-random bytes; 0: every operand byte is 1
+ * (register operands) and every out[t] equals 8192 * iters.  out holds
+ * blocks * 512 int32 (one per thread).
+ * Asynchronous on `stream`.  mgx_mfma_i8_ceiling_ops gives the INT8
+ * operations of one launch. */
+int mgx_mfma_i8_ceiling(int32_t *out, int blocks, int iters, int random_operands,
+                        void *stream);
+double mgx_mfma_i8_ceiling_ops(int blocks, int iters);
+
 /* ---- learned cost model ---------------------------------------------------
  * A 3-layer, width-64 FP32 MLP (in_dim -> 64 -> 64 -> 1, ReLU) trained and
  * evaluated on the GPU by the library's own kernels.  It replaces the fitted

@@ -87,6 +87,8 @@ _SIGNATURES = {
     "mgx_split_group_k": (_i64, [_int, _i64, _i64, _i64]),
     "mgx_summa_run": (_int, [ctypes.POINTER(SummaParams), _vp, _i64, _vp, _i64, _vp, _i64,
                              ctypes.POINTER(SummaStats)]),
+    "mgx_mfma_i8_ceiling": (_int, [_vp, _int, _int, _int, _vp]),
+    "mgx_mfma_i8_ceiling_ops": (_dbl, [_int, _int]),
     "mgx_host_alloc": (_int, [ctypes.POINTER(_vp), ctypes.c_size_t]),
     "mgx_host_free": (_int, [_vp]),
     "mgx_mlp_param_count": (_i64, [_int]),

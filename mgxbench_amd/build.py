@@ -23,7 +23,8 @@ BIN = PKG / "bin"
 LIB = PKG / "libmgx.so"
 INCLUDE = PKG.parent / "include"
 
-LIB_SOURCES = ["runtime.hip", "dgemm.hip", "ozaki.hip", "summa.hip", "accuracy.hip", "mlp.hip", "tune.hip"]
+LIB_SOURCES = ["runtime.hip", "dgemm.hip", "ozaki.hip", "summa.hip", "accuracy.hip", "mlp.hip", "tune.hip",
+               "ceiling.hip"]
 EXE_SOURCES = {"mm_1d": "mm_1d.cpp", "mm_2d": "mm_2d.cpp"}  # linked against libmgx.so
 
 ARCH = "gfx950"

@@ -61,3 +61,14 @@ def test_argument_errors_are_reported_without_a_device():
     assert st == _lib.MGX_ERR_UNSUPPORTED
     # empty problems are a successful no-op
     assert lib.mgx_dgemm(None, None, None, 0, 4, 4, 4, 4, 4, 32, None) == 0
+
+
+def test_mfma_ceiling_probe_host_side():
+    lib = _lib.lib()
+    # 8 waves x 32 MFMAs x 16 x 16 x 64 multiply-adds x 2 operations per round
+    assert lib.mgx_mfma_i8_ceiling_ops(1, 2) == 8 * 2 * 32 * 16 * 16 * 64 * 2
+    assert lib.mgx_mfma_i8_ceiling_ops(256, 12000) == 256 * 8 * 12000 * 32 * 32768
+    # odd round counts, empty grids and NULL outputs are refused before any launch
+    assert lib.mgx_mfma_i8_ceiling(None, 1, 2, 1, None) == _lib.MGX_ERR_INVALID_ARG
+    assert lib.mgx_mfma_i8_ceiling(ctypes.c_void_p(16), 1, 3, 1, None) == _lib.MGX_ERR_INVALID_ARG
+    assert lib.mgx_mfma_i8_ceiling(ctypes.c_void_p(16), 0, 2, 1, None) == _lib.MGX_ERR_INVALID_ARG
