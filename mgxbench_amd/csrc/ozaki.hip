@@ -1045,6 +1045,11 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     const SplitLayout L = split_layout(form, slices, m, n, k);
     const int64_t tiles = ((m + TBm - 1) / TBm) * ((n + TBn - 1) / TBn);
     MGX_ARG_CHECK(tiles <= 0x7fffffffll, "matrix too large for one launch");
+    // reconstruction: one thread per (tile, wave, fragment quad, lane)
+    const int recon_threads = tile == MGX_TILE_LARGE ? ShapeLarge::THREADS * ShapeLarge::QPL
+                                                     : ShapeSmall::THREADS * ShapeSmall::QPL;
+    const int64_t recon_blocks = (tiles * recon_threads + 255) / 256;
+    MGX_ARG_CHECK(!mod || recon_blocks <= 0x7fffffffll, "matrix too large for the reconstruction");
 
     char *ws = (char *)workspace;
     if (ws) {
@@ -1162,11 +1167,7 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     if ((rc = mgx::launch_status("int8_product_kernel")) != MGX_OK) return rc;
     if ((rc = mark(3)) != MGX_OK) return rc;
     if (mod) {
-        const int threads = tile == MGX_TILE_LARGE ? ShapeLarge::THREADS * ShapeLarge::QPL
-                                                   : ShapeSmall::THREADS * ShapeSmall::QPL;
-        const int64_t total = tiles * threads;
-        const int64_t blocks = (total + 255) / 256;
-        MGX_ARG_CHECK(blocks <= 0x7fffffffll, "matrix too large for the reconstruction");
+        const int64_t blocks = recon_blocks;
         // the common modulus counts get compile-time specialisations
         auto launch = [&](auto tb, auto nm) {
             constexpr int TBv = decltype(tb)::value, NMv = decltype(nm)::value;
