@@ -35,6 +35,7 @@ def _same(x, y):
 @pytest.mark.parametrize("m,n,k,nmod", [
     (1, 1, 1, 2), (16, 16, 64, 15), (17, 33, 5, 6), (130, 127, 129, 15), (128, 128, 256, 16),
     (200, 96, 1000, 13), (64, 300, 700, 15), (257, 129, 63, 4), (300, 260, 2000, 15),
+    (96, 150, 900, 14),
 ])
 def test_bit_exact_vs_model_uniform(mgx, orc, tile, m, n, k, nmod):
     announce(f"split mod uniform tile={tile} m={m} n={n} k={k} moduli={nmod}")
