@@ -72,3 +72,16 @@ def test_mfma_ceiling_probe_host_side():
     assert lib.mgx_mfma_i8_ceiling(None, 1, 2, 1, None) == _lib.MGX_ERR_INVALID_ARG
     assert lib.mgx_mfma_i8_ceiling(ctypes.c_void_p(16), 1, 3, 1, None) == _lib.MGX_ERR_INVALID_ARG
     assert lib.mgx_mfma_i8_ceiling(ctypes.c_void_p(16), 0, 2, 1, None) == _lib.MGX_ERR_INVALID_ARG
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    # no CPU fallback: without libmgx every entry point raises
+    code = ("import mgxbench_amd as mgx\n"
+            "try:\n"
+            "    mgx.lib()\n"
+            "except RuntimeError as e:\n"
+            "    print('raised', 'missing' in str(e))\n")
+    env = dict(__import__("os").environ, MGX_LIBRARY=str(tmp_path / "absent" / "libmgx.so"))
+    out = subprocess.run([__import__("sys").executable, "-c", code], capture_output=True,
+                         text=True, env=env, cwd=str(_lib.LIB_PATH.parents[1]), timeout=120)
+    assert out.stdout.strip() == "raised True", out.stdout + out.stderr
