@@ -11,9 +11,11 @@
 //
 // Split forms gather consecutive panels into a group buffer (whole panels
 // covering mgx_split_group_k columns) and run one split product per group:
-// the panel copies keep the reference's granularity and overlap the previous
-// group's product, while the exponent scan, operand conversion and (modular
-// form) reconstruction of the whole C are paid once per group.
+// the copies overlap the previous group's product, while the exponent scan,
+// operand conversion and (modular form) reconstruction of the whole C are
+// paid once per group.  With pinned strips and asynchronous copies a group
+// arrives as one A copy and one B copy; otherwise panel by panel, the
+// reference's granularity.
 //
 // MI355X-native differences from the reference's loop:
 //  * With pinned host strips the A panel is gathered by the DMA engine
@@ -155,36 +157,49 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
             const int64_t g0 = g * gw;
             const int64_t gk = (k - g0) < gw ? (k - g0) : gw;
             if (g >= 2) MGX_HIP_TRY(hipStreamWaitEvent(R.transfer, R.done[cur], 0));
-            for (int64_t j0 = 0; j0 < gk; j0 += panel, ++pi) {
-                const int64_t kc = (gk - j0) < panel ? (gk - j0) : panel;
-                const int64_t k0 = g0 + j0;
-                const double *srcA = hA + k0;
-                const double *srcB = hB + k0 * ldb;
-                int64_t spitchA = lda, spitchB = ldb;
-                const int s = (int)(pi & 1);
-                if (!pinned) {
-                    // pack_a / pack_b into the pinned staging buffer
-                    if (pi >= 2) MGX_HIP_TRY(hipEventSynchronize(R.staged[s]));
-                    for (int64_t i = 0; i < m; ++i)
-                        std::memcpy(R.hA[s] + i * kc, hA + i * lda + k0, sizeof(double) * kc);
-                    for (int64_t r = 0; r < kc; ++r)
-                        std::memcpy(R.hB[s] + r * n, hB + (k0 + r) * ldb, sizeof(double) * n);
-                    srcA = R.hA[s];
-                    srcB = R.hB[s];
-                    spitchA = kc;
-                    spitchB = n;
+            if (pinned && gk > panel && P->copy_mode == MGX_SUMMA_COPY_ASYNC && !P->serialize) {
+                // a split group's panels are adjacent columns of the A strip
+                // and adjacent rows of the B strip: one copy each (a copy
+                // costs ~50 us of idle copy engine whatever its size)
+                MGX_HIP_TRY(hipMemcpy2DAsync(R.dA[cur], sizeof(double) * gk, hA + g0,
+                                             sizeof(double) * lda, sizeof(double) * gk, m,
+                                             hipMemcpyHostToDevice, R.transfer));
+                MGX_HIP_TRY(hipMemcpy2DAsync(R.dB[cur], sizeof(double) * n, hB + g0 * ldb,
+                                             sizeof(double) * ldb, sizeof(double) * n, gk,
+                                             hipMemcpyHostToDevice, R.transfer));
+                pi += (gk + panel - 1) / panel;
+            } else {
+                for (int64_t j0 = 0; j0 < gk; j0 += panel, ++pi) {
+                    const int64_t kc = (gk - j0) < panel ? (gk - j0) : panel;
+                    const int64_t k0 = g0 + j0;
+                    const double *srcA = hA + k0;
+                    const double *srcB = hB + k0 * ldb;
+                    int64_t spitchA = lda, spitchB = ldb;
+                    const int s = (int)(pi & 1);
+                    if (!pinned) {
+                        // pack_a / pack_b into the pinned staging buffer
+                        if (pi >= 2) MGX_HIP_TRY(hipEventSynchronize(R.staged[s]));
+                        for (int64_t i = 0; i < m; ++i)
+                            std::memcpy(R.hA[s] + i * kc, hA + i * lda + k0, sizeof(double) * kc);
+                        for (int64_t r = 0; r < kc; ++r)
+                            std::memcpy(R.hB[s] + r * n, hB + (k0 + r) * ldb, sizeof(double) * n);
+                        srcA = R.hA[s];
+                        srcB = R.hB[s];
+                        spitchA = kc;
+                        spitchB = n;
+                    }
+                    // panel j0 of the group: columns j0.. of the m x gk A buffer
+                    // (lda = gk) and rows j0.. of the gk x n B buffer
+                    MGX_HIP_TRY(hipMemcpy2DAsync(R.dA[cur] + j0, sizeof(double) * gk, srcA,
+                                                 sizeof(double) * spitchA, sizeof(double) * kc, m,
+                                                 hipMemcpyHostToDevice, R.transfer));
+                    MGX_HIP_TRY(hipMemcpy2DAsync(R.dB[cur] + j0 * n, sizeof(double) * n, srcB,
+                                                 sizeof(double) * spitchB, sizeof(double) * n, kc,
+                                                 hipMemcpyHostToDevice, R.transfer));
+                    if (!pinned) MGX_HIP_TRY(hipEventRecord(R.staged[s], R.transfer));
+                    if (P->copy_mode == MGX_SUMMA_COPY_HOST_SYNC || P->serialize)
+                        MGX_HIP_TRY(hipStreamSynchronize(R.transfer));
                 }
-                // panel j0 of the group: columns j0.. of the m x gk A buffer
-                // (lda = gk) and rows j0.. of the gk x n B buffer
-                MGX_HIP_TRY(hipMemcpy2DAsync(R.dA[cur] + j0, sizeof(double) * gk, srcA,
-                                             sizeof(double) * spitchA, sizeof(double) * kc, m,
-                                             hipMemcpyHostToDevice, R.transfer));
-                MGX_HIP_TRY(hipMemcpy2DAsync(R.dB[cur] + j0 * n, sizeof(double) * n, srcB,
-                                             sizeof(double) * spitchB, sizeof(double) * n, kc,
-                                             hipMemcpyHostToDevice, R.transfer));
-                if (!pinned) MGX_HIP_TRY(hipEventRecord(R.staged[s], R.transfer));
-                if (P->copy_mode == MGX_SUMMA_COPY_HOST_SYNC || P->serialize)
-                    MGX_HIP_TRY(hipStreamSynchronize(R.transfer));
             }
             MGX_HIP_TRY(hipEventRecord(R.ready[cur], R.transfer));
             MGX_HIP_TRY(hipStreamWaitEvent(R.compute, R.ready[cur], 0));
