@@ -130,18 +130,19 @@ def _grouped(A, B, g, product):
 
 
 @pytest.mark.parametrize("split_k,group", [(0, 600), (256, 256), (300, 384), (100, 128)])
-@pytest.mark.parametrize("form_name", ["int8", "mod"])
+@pytest.mark.parametrize("form_name", ["int8", "mod", "bf16"])
 def test_split_forms_multiply_whole_panel_groups(mgx, orc, form_name, split_k, group):
     # panels of 128 (ragged last of 88); each group of whole panels is one
-    # split product, bit-exact against the oracle's model at that depth
+    # split product, bit-exact against the oracle's model at that depth (the
+    # bf16 form gives the INT8 form's result)
     m, n, k, panel = 190, 170, 600, 128
     announce(f"summa {form_name} split_k={split_k}")
-    form = mgx.FORM_INT8 if form_name == "int8" else mgx.FORM_INT8_MOD
+    form = {"int8": mgx.FORM_INT8, "mod": mgx.FORM_INT8_MOD, "bf16": mgx.FORM_BF16}[form_name]
     assert mgx.split_group_k(form, panel, k, split_k) == group
     assert mgx.split_group_k(mgx.FORM_FP64, panel, k, split_k) == panel
     A = orc.fill(m, k, 1, seed=41)
     B = orc.fill(k, n, 1, seed=42)
-    if form_name == "int8":
+    if form_name in ("int8", "bf16"):
         C, st = _run(mgx, A, B, panel, pinned=True, form=form, slices=8, split_k=split_k)
         ref = _grouped(A, B, group, lambda a, b, C: orc.split_gemm_int8(a, b, 8, C=C))
     else:
