@@ -1,5 +1,6 @@
 // FP64-accurate split-precision GEMM on the INT8 matrix cores of gfx950, in
-// two forms that share one INT8 product kernel.
+// two forms that share one INT8 product kernel, plus the digit-slice form's
+// bf16 variant (the same digits on the bf16 matrix cores, same result).
 //
 // No counterpart in the reference: both compute the same local product as
 // ABMultiply / ABMultiplyAccumulateAsync (1D-row-block/mulmat_kernel.cu:86,
@@ -33,8 +34,9 @@
 //     INT32 (reduced mod m_l between K chunks of at most 2046 tiles) and
 //     stores it mod m_l as one byte per output.
 //  3. A reconstruction kernel recovers the integer C' = A'B' from its n
-//     residues (direct Chinese remaindering), rounds it once to FP64 and writes
-//     C = ldexp(C', e_i + f_j - 2 beta) (or C_in + that).
+//     residues (direct Chinese remaindering in FP64 on exact 40-bit limbs),
+//     rounds it once to FP64 and writes C = ldexp(C', e_i + f_j - 2 beta)
+//     (or C_in + that).
 // The modular form needs n products instead of s(s+1)/2 and its only error
 // is the quantisation of A and B to beta bits.
 #include "mgx_internal.hpp"
