@@ -122,6 +122,8 @@ int mgx_validate_constant(const double *dC, int64_t rows, int64_t cols,
  * library-owned per-device buffer grown on demand.
  * Inputs must be finite; a row of A or column of B holding Inf/NaN
  * produces NaN in the corresponding row/column of C.
+ * Size limits (grid dimensions of the operand passes): m and k at most
+ * 4194048; larger calls return MGX_ERR_INVALID_ARG before any launch.
  * MGX_FORM_INT8_MOD: `slices` is the number of moduli (2..16); the result
  * is ldexp(RN(A'B'), e_i + f_j - 2 beta) with A', B' the 2^beta-scaled
  * integer operands (see mgx_split_mod_beta), so each output is rounded once.

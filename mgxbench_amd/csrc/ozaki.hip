@@ -1052,6 +1052,11 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
                                                      : ShapeSmall::THREADS * ShapeSmall::QPL;
     const int64_t recon_blocks = (tiles * recon_threads + 255) / 256;
     MGX_ARG_CHECK(!mod || recon_blocks <= 0x7fffffffll, "matrix too large for the reconstruction");
+    // operand passes: y = 64-row blocks of A, y = 64-k tiles of B
+    MGX_ARG_CHECK(L.Mp / 64 <= 65535 && L.KTC <= 0x7fffffffll, "m or k too large for the row pass");
+    MGX_ARG_CHECK(L.Kp / KT <= 65535 && L.Np / 64 <= 0x7fffffffll,
+                  "k or n too large for the column pass");
+    MGX_ARG_CHECK((k + 255) / 256 <= 65535, "k too large for the column scan");
 
     char *ws = (char *)workspace;
     if (ws) {
@@ -1089,7 +1094,6 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     {
         const unsigned gx = (unsigned)((n + 255) / 256);
         const int64_t gy = (k + 255) / 256;
-        MGX_ARG_CHECK(gy <= 65535, "k too large for the column scan");
         hipLaunchKernelGGL(col_absmax_kernel, dim3(gx, (unsigned)gy), dim3(256), 0, st, B, k, n,
                            ldb, cmax);
         if ((rc = mgx::launch_status("col_absmax_kernel")) != MGX_OK) return rc;
@@ -1102,7 +1106,6 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
 
     {
         const int64_t gy = L.Mp / 64;
-        MGX_ARG_CHECK(gy <= 65535 && L.KTC <= 0x7fffffffll, "m or k too large for the row pass");
         // one workgroup per 64 k values and 64 lines
         const dim3 grid((unsigned)(L.Kp / KT), (unsigned)gy);
         if (bf)
@@ -1118,8 +1121,6 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     }
     {
         // x: 64-column blocks of B, y: k tiles
-        MGX_ARG_CHECK(L.Kp / KT <= 65535 && L.Np / 64 <= 0x7fffffffll,
-                      "k or n too large for the column pass");
         const dim3 grid((unsigned)(L.Np / 64), (unsigned)(L.Kp / KT));
         if (bf)
             hipLaunchKernelGGL((operand_tile_kernel<false, true, true>), grid, dim3(256), 0, st,

@@ -85,3 +85,15 @@ def test_missing_library_fails_loudly(tmp_path):
     out = subprocess.run([__import__("sys").executable, "-c", code], capture_output=True,
                          text=True, env=env, cwd=str(_lib.LIB_PATH.parents[1]), timeout=120)
     assert out.stdout.strip() == "raised True", out.stdout + out.stderr
+
+
+def test_split_size_limits_refused_before_any_launch():
+    # m or k beyond 4194048 exceed the operand passes' grid: the call is
+    # refused during argument checks (fake pointers are never dereferenced)
+    lib = _lib.lib()
+    p = ctypes.c_void_p(256)
+    for m, k in ((5_000_000, 64), (64, 5_000_000)):
+        st = lib.mgx_split_gemm(_lib.FORM_INT8_MOD, 15, p, p, p, m, 16, k, k, 16, 16, 0, None, 0,
+                                None)
+        assert st == _lib.MGX_ERR_INVALID_ARG, (m, k)
+        assert b"too large" in lib.mgx_last_error()
