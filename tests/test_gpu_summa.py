@@ -169,6 +169,40 @@ def test_split_groups_agree_across_streaming_modes(mgx, orc):
         assert np.array_equal(C0, C1), extra
 
 
+def test_padded_host_strips_grouped_copies(mgx, orc):
+    # pinned strips with lda > k and ldb > n: each split group arrives as one
+    # strided A copy and one B copy; the pageable run copies panel by panel;
+    # both equal the oracle's grouped model bit for bit
+    m, n, k, panel, lda, ldb = 150, 140, 700, 128, 700 + 24, 140 + 8
+    announce("summa modular groups: padded host strips")
+    A = orc.fill(m, k, 1, seed=47)
+    B = orc.fill(k, n, 1, seed=48)
+    ref = _grouped(A, B, 256, lambda a, b, C: orc.crt_gemm(a, b, 15, C=C))
+    pa, pb = PinnedArray(mgx, m, lda), PinnedArray(mgx, k, ldb)
+    try:
+        pa.array[:] = 7.0
+        pb.array[:] = 7.0
+        pa.array[:, :k] = A
+        pb.array[:, :n] = B
+        dC = empty(m, n, fill=0.0)
+        st = mgx.summa_run(pa.ptr, lda, pb.ptr, ldb, dC, n, m=m, n=n, k=k, panel=panel,
+                           form=mgx.FORM_INT8_MOD, slices=15, split_k=256)
+        C_pinned = to_host(dC)
+    finally:
+        pa.free()
+        pb.free()
+    assert st["panels"] == 6 and st["host_pinned"] == 1
+    Ap = np.full((m, lda), 7.0)
+    Bp = np.full((k, ldb), 7.0)
+    Ap[:, :k] = A
+    Bp[:, :n] = B
+    dC = empty(m, n, fill=0.0)
+    mgx.summa_run(Ap.ctypes.data, lda, Bp.ctypes.data, ldb, dC, n, m=m, n=n, k=k, panel=panel,
+                  form=mgx.FORM_INT8_MOD, slices=15, split_k=256)
+    assert np.array_equal(C_pinned, ref)
+    assert np.array_equal(to_host(dC), ref)
+
+
 def test_invalid_summa_arguments(mgx):
     A = np.zeros((4, 4))
     dC = empty(4, 4)
