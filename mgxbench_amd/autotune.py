@@ -107,9 +107,15 @@ def rank_shares(sizes=SIZES) -> list[tuple[int, int, int]]:
     return shapes
 
 
-def random_shapes(count: int, seed: int = 7, sizes=SIZES) -> list[tuple[int, int, int]]:
+def random_shapes(count: int, seed: int = 7, sizes=SIZES,
+                  exclude=()) -> list[tuple[int, int, int]]:
+    """`count` distinct (m, n, k) drawn from sizes^3, none of them in `exclude`."""
+    seen = {tuple(s) for s in exclude}
+    if count > len(sizes) ** 3 - len(seen & {(a, b, c) for a in sizes for b in sizes
+                                              for c in sizes}):
+        raise ValueError("not enough distinct shapes")
     rng = random.Random(seed)
-    seen, out = set(), []
+    out = []
     while len(out) < count:
         s = (rng.choice(sizes), rng.choice(sizes), rng.choice(sizes))
         if s not in seen:
@@ -438,6 +444,10 @@ def main(argv=None) -> int:
     p.add_argument("--max-size", type=int, default=32768)
     p.add_argument("--time-budget", type=float, default=None,
                    help="seconds; stop before a shape that would overrun it")
+    p.add_argument("--seed", type=int, default=7, help="random-shape seed")
+    p.add_argument("--no-shares", action="store_true", help="skip the grid rank shares")
+    p.add_argument("--exclude-dataset", default=None,
+                   help="draw random shapes outside this dataset's shapes (a held-out set)")
     p = sub.add_parser("train")
     p.add_argument("--dataset", required=True)
     p.add_argument("--out", required=True)
@@ -448,6 +458,10 @@ def main(argv=None) -> int:
     p.add_argument("--dataset", required=True)
     p.add_argument("--model", required=True)
     p.add_argument("--out", default=None)
+    p.add_argument("--all-heldout", action="store_true",
+                   help="every shape of --dataset is held out (a separately measured set)")
+    p.add_argument("--train-dataset", default=None,
+                   help="the training rows, for the power-law baseline of --all-heldout")
     p = sub.add_parser("select")
     p.add_argument("--model", required=True)
     p.add_argument("--m", type=int, required=True)
@@ -470,7 +484,11 @@ def main(argv=None) -> int:
         return share(a)
     if a.cmd == "measure":
         sizes = [s for s in SIZES if s <= a.max_size]
-        shapes = rank_shares(sizes) + random_shapes(a.random_shapes, sizes=sizes)
+        exclude = []
+        if a.exclude_dataset:
+            exclude = json.loads(Path(a.exclude_dataset).read_text())["shapes_done"]
+        shapes = ([] if a.no_shares else rank_shares(sizes)) + random_shapes(
+            a.random_shapes, seed=a.seed, sizes=sizes, exclude=exclude)
         doc = measure(shapes, Path(a.out), time_budget_s=a.time_budget)
         print(f"[autotune] {len(doc['rows'])} measured configurations in {a.out}")
     elif a.cmd == "train":
@@ -492,7 +510,14 @@ def main(argv=None) -> int:
         model = MLPModel.from_json(json.loads(Path(a.model).read_text()))
         tr = [tuple(s) for s in model.meta["train_shapes"]]
         ho = [tuple(s) for s in model.meta["heldout_shapes"]]
-        res = evaluate(doc["rows"], model, tr, ho)
+        rows = doc["rows"]
+        if a.all_heldout:
+            ho = sorted({(r["m"], r["n"], r["k"]) for r in rows})
+            if set(ho) & set(tr):
+                ap.error("--all-heldout: the dataset shares shapes with the model's training set")
+            if a.train_dataset:
+                rows = rows + json.loads(Path(a.train_dataset).read_text())["rows"]
+        res = evaluate(rows, model, tr, ho)
         text = json.dumps(res, indent=2)
         if a.out:
             Path(a.out).write_text(text)

@@ -137,3 +137,16 @@ def test_evaluate_regret_bookkeeping(monkeypatch):
 def test_share_shape():
     assert at.share_shape("1d", 32768, 4) == (8192, 32768, 32768)
     assert at.share_shape("2d", 16384, 4, (2, 2)) == (8192, 8192, 16384)
+
+
+def test_random_shapes_exclusion_for_heldout_sets():
+    from mgxbench_amd import autotune as at
+    train = at.rank_shares() + at.random_shapes(56)
+    held = at.random_shapes(200, seed=8, exclude=train)
+    assert len(set(held)) == 200 and not set(held) & set(train)
+    assert held == at.random_shapes(200, seed=8, exclude=train)       # deterministic
+    assert at.random_shapes(56) == train[24:]                          # default stream unchanged
+    space = len(at.SIZES) ** 3
+    with pytest.raises(ValueError):                                    # one shape of the space excluded
+        at.random_shapes(space, exclude=at.random_shapes(1))
+    assert len(at.random_shapes(space, exclude=[(1, 2, 3)])) == space  # outside the space: no effect
