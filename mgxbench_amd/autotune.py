@@ -126,7 +126,22 @@ def random_shapes(count: int, seed: int = 7, sizes=SIZES,
 
 FEATURE_NAMES = ["log2_m", "log2_n", "log2_k", "log2_panel", "log2_panels", "ragged_frac",
                  "int8", "slices", "products", "tile_small", "log2_mnk", "log2_mn", "mod",
-                 "log2_step_k", "log2_steps"]
+                 "log2_step_k", "log2_steps", "wave_fill"]
+
+# C block tile edge and concurrently resident workgroups (CU count x
+# workgroups per CU) of each product kernel, for the wave-fill feature
+_TILE_SLOTS = {("split", False): (256, 256), ("split", True): (128, 512),
+               ("fp64", False): (128, 512), ("fp64", True): (64, 1024)}
+
+
+def wave_fill(m: int, n: int, c) -> float:
+    """Mean occupancy of the workgroup rounds of one product launch:
+    (tiles / slots) / ceil(tiles / slots), 1 when the last round is full."""
+    edge, slots = _TILE_SLOTS[("fp64" if c.form == "fp64" else "split",
+                               c.tile == _lib.TILE_SMALL)]
+    tiles = math.ceil(m / edge) * math.ceil(n / edge)
+    rounds = math.ceil(tiles / slots)
+    return tiles / (slots * rounds)
 
 
 def features(m: int, n: int, k: int, c: Candidate) -> list[float]:
@@ -143,7 +158,7 @@ def features(m: int, n: int, k: int, c: Candidate) -> list[float]:
             1.0 if c.form == "int8" else 0.0, c.slices / 8.0, products / 36.0,
             1.0 if c.tile == _lib.TILE_SMALL else 0.0, math.log2(m * n * k) / 3.0,
             math.log2(m * n) / 2.0, 1.0 if c.form == "mod" else 0.0,
-            math.log2(sk), math.log2(steps)]
+            math.log2(sk), math.log2(steps), wave_fill(m, n, c)]
 
 
 # ---- measurement (GPU) --------------------------------------------------------------------

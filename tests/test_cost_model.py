@@ -11,6 +11,11 @@ autotuning/predictive.py:100-127 is reported beside it.
 
 The CPU test evaluates the network with the oracle's double-precision
 forward pass; the GPU test with the library's own FP32 inference kernel.
+
+mgxbench_amd/models/heldout_dataset.json holds 200 further shapes, drawn
+outside the training dataset and measured separately (every candidate of
+each): there the network's first pick must stay within 5% of the best on
+every shape, and its mean regret must beat the power-law fit's.
 """
 import json
 from pathlib import Path
@@ -77,3 +82,16 @@ def test_heldout_regret_with_the_gpu_forward():
     rows, model, tr, ho = _load()
     res = at.evaluate(rows, model, tr, ho)
     _check(res, model.meta["metrics"])
+
+
+def test_separate_heldout_set_with_the_oracle_forward(orc):
+    rows, model, tr, _ = _load()
+    doc = json.loads((MODELS / "heldout_dataset.json").read_text())
+    ho = sorted({(r["m"], r["n"], r["k"]) for r in doc["rows"]})
+    assert len(ho) == 200 and not set(ho) & set(tr)
+    res = at.evaluate(rows + doc["rows"], _OracleForward(model, orc), tr, ho)
+    assert res["heldout_shapes"] == 200
+    assert res["regret_mlp_max"] <= 0.05
+    assert res["regret_mlp_mean"] < res["regret_powerlaw_mean"]
+    assert res["regret_top3_mean"] <= res["regret_mlp_mean"] + 1e-12
+    assert res["mape"] <= 0.06
