@@ -159,6 +159,21 @@ def summa_report(mgx, torch, A, B, m, n, k, nmod) -> dict:
 
 
 
+def vendor_fp64(torch, A, B, stream, timed, warmup, steps) -> dict:
+    """Sanity roof for the plain denominator: the vendor FP64 GEMM (torch.matmul
+    in float64, which dispatches to hipBLASLt / rocBLAS) on the same operands."""
+    with torch.cuda.stream(stream):
+        out = torch.empty((A.shape[0], B.shape[1]), dtype=torch.float64, device=A.device)
+        for _ in range(warmup):
+            torch.matmul(A, B, out=out)
+        torch.cuda.synchronize()
+        ms = timed(lambda i: torch.matmul(A, B, out=out), steps)
+    F = 2.0 * A.shape[0] * B.shape[1] * A.shape[1]
+    del out
+    return {"library": "torch.matmul float64 (hipBLASLt / rocBLAS)", "ms_per_step": round(ms, 4),
+            "tflops": round(F / (ms * 1e-3) / 1e12, 3)}
+
+
 def mfma_ceiling(mgx, torch, stream, iters=12000, reps=3):
     """INT8 matrix-core rate of an MFMA-only loop shaped like the product
     kernel (mgx_mfma_i8_ceiling: one 8-wave workgroup per CU), with random
@@ -247,6 +262,8 @@ def main() -> int:
     rows = list(range(0, m, max(1, m // 16)))[:16]
     acc_plain = mgx.error_report(A, B, Cp, m, n, k, k, n, n, rows)
     acc_plain.pop("rows_checked", None)
+    vendor = vendor_fp64(torch, A, B, stream, timed, args.warmup, max(3, args.steps // 4))
+    log(f"vendor fp64 {vendor['ms_per_step']:.3f} ms/step ({vendor['tflops']:.1f} TFLOP/s)")
 
     def measure(name):
         form, planes = forms[name]
@@ -326,6 +343,8 @@ def main() -> int:
             "tflops": round(tf_plain, 3),
             "ms_per_step": round(ms_plain, 4),
             "frac_of_fp64_mfma_peak": round(tf_plain / mgx.peak(mgx.FORM_FP64), 4),
+            "vendor_fp64_roof": vendor,
+            "vs_vendor_fp64": round(vendor["ms_per_step"] / ms_plain, 4),
         },
         "speedup_vs_plain": round(tf_split / tf_plain, 4),
         "split_phase_ms": {p: round(v, 4) for p, v in head["phases"].items()},

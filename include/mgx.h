@@ -10,9 +10,13 @@
  * Threading: entry points may be called from any host thread; calls that
  * share a stream are ordered by that stream.  The library keeps per-device
  * scratch allocations (validation partials, split-form workspaces grown on
- * demand) that are reused across calls on the same device: two calls that
- * use the library-owned split workspace on one device must be ordered on
- * one stream (they race otherwise).
+ * demand) that are reused across calls on the same device.  A call that uses
+ * the library-owned split workspace on another stream than the previous such
+ * call waits (hipStreamWaitEvent) for that call's work first, so calls issued
+ * from one host thread are ordered as issued; calls from several host threads
+ * at once must bring their own workspaces.  Inside a stream capture the
+ * library scratch can neither grow nor change streams: such a call returns
+ * MGX_ERR_WORKSPACE (pass a workspace instead).
  */
 #ifndef MGX_H
 #define MGX_H
@@ -30,7 +34,8 @@ enum {
     MGX_ERR_INVALID_ARG = 1,   /* a size, leading dimension or pointer is invalid */
     MGX_ERR_HIP = 2,           /* a HIP runtime call failed (see mgx_last_error) */
     MGX_ERR_UNSUPPORTED = 3,   /* form / slice count / option not supported */
-    MGX_ERR_WORKSPACE = 4,     /* caller workspace too small */
+    MGX_ERR_WORKSPACE = 4,     /* caller workspace too small, or library scratch not usable
+                                  inside a stream capture */
     MGX_ERR_ALLOC = 5          /* device or pinned allocation failed */
 };
 

@@ -9,9 +9,12 @@ fit ("ml", :140-165) or a hold-out vote between them ("auto", :213-233).
 Here the candidates are the knobs of the MI355X local product, which the
 reference does not have: form (FP64 MFMA, INT8 digit-slice split with s
 slices, or INT8 modular split with n moduli), the kernel block tile and the
-SUMMA panel width.  Only FP64-accurate candidates are offered: s >= 7 slices,
-and moduli counts whose scaling exponent at the panel depth is at least
-MIN_BETA (mgx_split_mod_beta).  One MLP (3 layers, width 64,
+SUMMA panel width.  Only FP64-accurate candidates are offered, calibrated on
+the accuracy-stress inputs (random sign x U[1,2) x 2^U(-30,30), rows / columns
+x 2^U(-10,10)) as well as on uniform ones: s = 8 slices (s = 7 is up to 1.9x
+plain FP64's componentwise error on stress inputs at depth 4096), and moduli
+counts whose scaling exponent at the split-product depth is at least MIN_BETA
+(mgx_split_mod_beta): 15 moduli at group depths up to 4096, 16 at any depth.  One MLP (3 layers, width 64,
 FP32, trained on the GPU by libmgx's own kernels) predicts log runtime from
 log-shape features for every candidate at once; exhaustive timing of its
 top-k candidates stays the fallback and the arbiter.
@@ -47,9 +50,11 @@ from . import _lib
 
 SIZES = list(range(4096, 32768 + 1, 4096))
 PANELS = (128, 256, 512, 1024)
-SLICES = (7, 8)
-MODULI = (14, 15, 16)
-MIN_BETA = 49   # quantisation to >= 49 bits keeps the modular form within plain FP64's error
+SLICES = (8,)
+MODULI = (15, 16)
+# quantisation to >= 52 bits per row / column keeps the modular form within
+# plain FP64's normwise and componentwise error on stress inputs
+MIN_BETA = 52
 TILES = (_lib.TILE_LARGE, _lib.TILE_SMALL)
 DEFAULT = None  # filled below: the reference's default configuration
 SCHEMA = 1
@@ -340,10 +345,15 @@ class PowerLawModel:
 
 
 def evaluate(rows, model: MLPModel, train_shapes, heldout_shapes, topk: int = 3) -> dict:
-    """Held-out prediction error and regret (measured t(pick) / t(best) - 1)."""
+    """Held-out prediction error and regret (measured t(pick) / t(best) - 1)
+    over the current candidate space (rows of retired candidates are skipped)."""
     by_shape = {}
+    space = {}
     for r in rows:
-        by_shape.setdefault((r["m"], r["n"], r["k"]), []).append(r)
+        if r["k"] not in space:
+            space[r["k"]] = set(candidate_space(r["k"]))
+        if Candidate(r["form"], r["slices"], r["tile"], r["panel"]) in space[r["k"]]:
+            by_shape.setdefault((r["m"], r["n"], r["k"]), []).append(r)
     base = PowerLawModel(rows, train_shapes)
     ape, regret_mlp, regret_topk, regret_base, regret_default, per_shape = [], [], [], [], [], []
     for s in heldout_shapes:

@@ -104,7 +104,7 @@ extern "C" int mgx_error_report(const double *A, const double *B, const double *
     const size_t rows_bytes = ((sizeof(int64_t) * nrows + 255) / 256) * 256;
     int st = MGX_OK;
     char *scratch = (char *)mgx::device_scratch(
-        mgx::SCRATCH_VALIDATE, rows_bytes + sizeof(ErrPartial) * nblocks, &st);
+        mgx::SCRATCH_VALIDATE, rows_bytes + sizeof(ErrPartial) * nblocks, nullptr, &st);
     if (!scratch) return st;
     int64_t *d_rows = (int64_t *)scratch;
     ErrPartial *d_part = (ErrPartial *)(scratch + rows_bytes);
@@ -115,6 +115,7 @@ extern "C" int mgx_error_report(const double *A, const double *B, const double *
     std::vector<ErrPartial> host(nblocks);
     MGX_HIP_TRY(hipMemcpy(host.data(), d_part, sizeof(ErrPartial) * nblocks,
                           hipMemcpyDeviceToHost));
+    if ((st = mgx::scratch_release(mgx::SCRATCH_VALIDATE, nullptr)) != MGX_OK) return st;
     double e2 = 0.0, f2 = 0.0, comp = 0.0, mabs = 0.0;
     for (const auto &p : host) {
         e2 += p.err2;

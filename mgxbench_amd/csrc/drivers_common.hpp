@@ -79,6 +79,39 @@ inline bool parse_options(Options *o) {
     return true;
 }
 
+// True when any knob of this build that changes the workload or its timing is
+// set.  Only then do the executables append their extra keys (form, slices,
+// share, panels) to the BENCHMARK_RESULT line; with none set the line is the
+// reference's format string byte for byte (1D-row-block/mmb.cu:964,
+// 2D-block-matrix/mmb.cu:398).
+inline bool extras_requested() {
+    const char *knobs[] = {"MGX_FORM", "MGX_SLICES", "MGX_INIT", "MGX_SHARE_RANKS",
+                           "MGX_SHARE_GRID", "MGX_SPLIT_K", "MGX_SUMMA_GRAPH",
+                           "MGX_HOST_PINNED", "MGX_SUMMA_SERIALIZE", "MGX_RESULT_EXTRAS"};
+    for (const char *k : knobs) {
+        const char *v = std::getenv(k);
+        if (v && *v) return true;
+    }
+    return false;
+}
+
+// Test hook MGX_TEST_PERTURB=<index>:<value>: overwrite element `index` of the
+// `count` contiguous doubles of the device result C with `value` (strtod
+// syntax, so "nan" and "inf" work) after the timed region and before
+// validation, so the failing-validation path can be exercised end to end.
+// Returns false on a malformed value or an index outside C.
+inline bool apply_test_perturbation(double *dC, int64_t count) {
+    const char *s = std::getenv("MGX_TEST_PERTURB");
+    if (!s || !*s) return true;
+    char *end = nullptr;
+    const long long idx = std::strtoll(s, &end, 10);
+    if (end == s || *end != ':' || idx < 0 || idx >= count) return false;
+    const char *vs = end + 1;
+    const double value = std::strtod(vs, &end);
+    if (end == vs || *end != '\0') return false;
+    return hipMemcpy(dC + idx, &value, sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+}
+
 inline const char *form_name(int form) {
     return form == MGX_FORM_INT8       ? "int8"
            : form == MGX_FORM_INT8_MOD ? "mod"

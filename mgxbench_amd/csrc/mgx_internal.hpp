@@ -36,8 +36,14 @@ inline int launch_status(const char *what) {
 }
 
 // Per-device library-owned device buffer, grown on demand and kept until exit.
-// Not thread safe across calls that share a device without a common stream.
-void *device_scratch(int slot, size_t bytes, int *status);
+// `stream` is the stream the caller's work on the buffer will be queued on:
+// when the previous user was another stream, that stream's last use is
+// waited for first; growing it inside a stream capture, or switching streams
+// inside one, fails with MGX_ERR_WORKSPACE (the caller must pass a workspace).
+// Every successful acquire is followed by scratch_release(slot, stream) once
+// the work has been queued.
+void *device_scratch(int slot, size_t bytes, void *stream, int *status);
+int scratch_release(int slot, void *stream);
 
 enum ScratchSlot { SCRATCH_VALIDATE = 0, SCRATCH_SPLIT = 1, SCRATCH_MAX = 2 };
 

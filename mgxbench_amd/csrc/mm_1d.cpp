@@ -18,9 +18,13 @@
 // The pipeline double-buffers chunk stages with "B ready" / "compute done"
 // events and runs C += A[:, k0:k0+kc] . B_chunk per chunk on a compute stream.
 //
-// Extensions (new env vars): MGX_FORM=fp64|int8|mod, MGX_SLICES, MGX_INIT=
+// Extensions (new env vars): MGX_FORM=fp64|int8|bf16|mod, MGX_SLICES, MGX_INIT=
 // uniform:SEED (validated by the GPU double-double check), MGX_SHARE_RANKS=P
-// (rank 0's row block of a P-rank run: A_i is N/P x N).
+// (rank 0's row block of a P-rank run: A_i is N/P x N), MGX_SPLIT_K.  Only
+// when one of them is set does the result line carry the extra keys form=,
+// slices= and share_ranks=; otherwise it is the reference's line exactly.
+// MGX_TEST_PERTURB=<index>:<value> (test hook) overwrites one element of C
+// before validation.
 #include <chrono>
 #include <vector>
 
@@ -222,6 +226,7 @@ int main(int argc, char **argv) {
     std::printf("\n\nmmb myRank=0\tLibraries=%s\t RESULT: Matrix size: %d\tTime (seconds): %.9f\n\n",
                 libs, matrix_size, avg);
 
+    if (!apply_test_perturbation(d_C, mi * n)) fail("invalid MGX_TEST_PERTURB");
     int valid = 0;
     if (opt.uniform) {
         double nw = 0.0;
@@ -234,9 +239,11 @@ int main(int argc, char **argv) {
         valid = pass;
     }
     std::printf("BENCHMARK_RESULT decomposition=1D library=%s size=%d ranks=%d seconds=%.9f valid=%d "
-                "loops=%d form=%s slices=%d share_ranks=%d\n",
-                libs, matrix_size, nranks, avg, valid, loop_count, form_name(opt.form), opt.slices,
-                share);
+                "loops=%d",
+                libs, matrix_size, nranks, avg, valid, loop_count);
+    if (extras_requested())
+        std::printf(" form=%s slices=%d share_ranks=%d", form_name(opt.form), opt.slices, share);
+    std::printf("\n");
     std::fflush(stdout);
 
     if (pipelined) {

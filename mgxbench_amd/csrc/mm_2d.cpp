@@ -6,20 +6,27 @@
 // BENCHMARK_RESULT line; exit 0 when valid, 1 otherwise.
 //
 // The rank's local update C_ij += sum_p A_ik(p) . B_kj(p) runs through
-// mgx_summa_run: panels stream from pinned host strips by hipMemcpy2DAsync on
-// a transfer stream, overlapped with the accumulate product on a compute
-// stream (the single-GPU stand-in for the per-panel broadcasts).  With one
-// rank, every library code moves panels host -> device; MMM/YYY/NNN overlap
-// the copies, CCC/XXX/WWW/SSS wait for each copy on the host, as the
+// mgx_summa_run: panels stream from the host strips on a transfer stream,
+// overlapped with the accumulate product on a compute stream (the single-GPU
+// stand-in for the per-panel broadcasts).  As in the reference's timed loop
+// (2D-block-matrix/mmb.cu:264-265), the strips are pageable and every panel
+// is packed on the host into a pinned staging buffer before its copy.  With
+// one rank, every library code moves panels host -> device; MMM/YYY/NNN
+// overlap the copies, CCC/XXX/WWW/SSS wait for each copy on the host, as the
 // reference's P=1 paths do.
 //
-// Extensions (new env vars; the CLI and the result keys the runner checks are
-// unchanged): MGX_FORM=fp64|int8|mod and MGX_SLICES select the product form;
-// MGX_INIT=uniform:SEED replaces the constant A=1, B=2 inputs (validated by
-// the GPU double-double check instead of the constant rule);
-// MGX_SHARE_GRID=RxC computes rank (0,0)'s share of an R x C grid at this N
-// (m = N/R, n = N/C, K = N); MGX_SUMMA_GRAPH=1 captures the panel loop in a
-// hipGraph; MGX_HOST_PINNED=0 keeps pageable strips (host packing).
+// Extensions (new env vars; with none of them set the result line is the
+// reference's byte for byte, otherwise it carries the extra keys form=,
+// slices=, share= and panels=): MGX_FORM=fp64|int8|bf16|mod and MGX_SLICES
+// select the product form; MGX_INIT=uniform:SEED replaces the constant A=1,
+// B=2 inputs (validated by the GPU double-double check instead of the
+// constant rule); MGX_SHARE_GRID=RxC computes rank (0,0)'s share of an R x C
+// grid at this N (m = N/R, n = N/C, K = N); MGX_SUMMA_GRAPH=1 captures the
+// panel loop in a hipGraph (with pinned strips); MGX_HOST_PINNED=1 allocates
+// the strips pinned, so panels are gathered by strided DMA without host
+// packing.
+// MGX_TEST_PERTURB=<index>:<value> (test hook) overwrites one element of the
+// local C before validation.
 #include <cmath>
 #include <vector>
 
@@ -78,14 +85,17 @@ int main(int argc, char **argv) {
     if (!parse_options(&opt)) fail("invalid MGX_FORM or MGX_INIT");
     int dev = 0;
     if (select_device(argv[1], &dev) != 0) fail("GPU device mapping invalid");
-    const bool pinned = env_int("MGX_HOST_PINNED", 1) != 0;
     const bool graph = env_int("MGX_SUMMA_GRAPH", 0) != 0;
+    // a captured panel loop gathers its panels by DMA from pinned strips
+    const bool pinned = env_int("MGX_HOST_PINNED", graph ? 1 : 0) != 0;
 
     std::printf("SUMMA rectangular grid=%dx%d N=%d BK=%d library=%s ranks=%d pipeline=2\n", Pr, Pc, N,
                 BK, lib, p);
-    std::printf("form=%s slices=%d share=%dx%d local C %lldx%lld host strips %s%s\n",
-                form_name(opt.form), opt.slices, Sr, Sc, (long long)m, (long long)n,
-                pinned ? "pinned" : "pageable", graph ? " hipGraph" : "");
+    const bool extras = extras_requested();
+    if (extras)
+        std::printf("form=%s slices=%d share=%dx%d local C %lldx%lld host strips %s%s\n",
+                    form_name(opt.form), opt.slices, Sr, Sc, (long long)m, (long long)n,
+                    pinned ? "pinned" : "pageable", graph ? " hipGraph" : "");
 
     // Host strips: A_ik (m x N) and B_kj (N x n) of the rank's row / column.
     const size_t a_elems = (size_t)m * N, b_elems = (size_t)N * n;
@@ -128,6 +138,7 @@ int main(int argc, char **argv) {
     mgx_summa_stats S;
     if (mgx_summa_run(&P, hA, N, hB, n, dC, n, &S) != MGX_OK) fail(mgx_last_error());
     const double elapsed = S.seconds;
+    if (!apply_test_perturbation(dC, m * n)) fail("invalid MGX_TEST_PERTURB");
 
     int valid = 1;
     double normwise = -1.0;
@@ -161,9 +172,12 @@ int main(int argc, char **argv) {
         }
     }
     std::printf("BENCHMARK_RESULT decomposition=2D library=%s size=%d ranks=%d seconds=%.9f valid=%d "
-                "loops=1 grid=%dx%d form=%s slices=%d share=%dx%d panels=%lld\n",
-                lib, N, p, elapsed, valid, Pr, Pc, form_name(opt.form), opt.slices, Sr, Sc,
-                (long long)S.panels);
+                "loops=1 grid=%dx%d",
+                lib, N, p, elapsed, valid, Pr, Pc);
+    if (extras)
+        std::printf(" form=%s slices=%d share=%dx%d panels=%lld", form_name(opt.form), opt.slices,
+                    Sr, Sc, (long long)S.panels);
+    std::printf("\n");
     std::fflush(stdout);
     (void)hipFree(dC);
     if (pinned) {

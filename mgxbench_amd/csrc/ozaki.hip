@@ -1059,6 +1059,14 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     MGX_ARG_CHECK((k + 255) / 256 <= 65535, "k too large for the column scan");
 
     char *ws = (char *)workspace;
+    // marks the library scratch as used by `st` once this call's work is queued
+    struct ScratchUse {
+        bool owned = false;
+        void *stream = nullptr;
+        ~ScratchUse() {
+            if (owned) (void)mgx::scratch_release(mgx::SCRATCH_SPLIT, stream);
+        }
+    } scratch_use;
     if (ws) {
         if (workspace_bytes < L.total)
             return mgx::set_error(MGX_ERR_WORKSPACE, "split workspace too small");
@@ -1066,8 +1074,10 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
             return mgx::set_error(MGX_ERR_INVALID_ARG, "workspace must be 256-byte aligned");
     } else {
         int status = MGX_OK;
-        ws = (char *)mgx::device_scratch(mgx::SCRATCH_SPLIT, L.total, &status);
+        ws = (char *)mgx::device_scratch(mgx::SCRATCH_SPLIT, L.total, stream, &status);
         if (!ws) return status;
+        scratch_use.owned = true;
+        scratch_use.stream = stream;
     }
     int8_t *Aop = (int8_t *)(ws + L.a_off);
     int8_t *Bop = (int8_t *)(ws + L.b_off);

@@ -22,13 +22,26 @@ namespace {
 struct ScratchEntry {
     void *ptr = nullptr;
     size_t bytes = 0;
+    hipStream_t last_stream = nullptr;   // stream of the last call that used the buffer
+    hipEvent_t last_use = nullptr;       // recorded on last_stream after that call's work
+    bool used = false;
 };
 std::mutex g_scratch_mu;
 std::vector<ScratchEntry> g_scratch[SCRATCH_MAX];
+
+bool is_capturing(hipStream_t stream) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return cs != hipStreamCaptureStatusNone;
+}
 }  // namespace
 
-void *device_scratch(int slot, size_t bytes, int *status) {
+void *device_scratch(int slot, size_t bytes, void *stream_ptr, int *status) {
     *status = MGX_OK;
+    hipStream_t stream = (hipStream_t)stream_ptr;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) {
         *status = set_error(MGX_ERR_HIP, "hipGetDevice failed");
@@ -38,24 +51,69 @@ void *device_scratch(int slot, size_t bytes, int *status) {
     auto &vec = g_scratch[slot];
     if ((int)vec.size() <= dev) vec.resize(dev + 1);
     ScratchEntry &e = vec[dev];
-    if (e.bytes >= bytes && e.ptr) return e.ptr;
-    if (e.ptr) {
-        // The previous buffer may still be referenced by queued work.
-        if (hipDeviceSynchronize() != hipSuccess || hipFree(e.ptr) != hipSuccess) {
-            *status = set_error(MGX_ERR_HIP, "releasing scratch failed");
+    const bool capturing = is_capturing(stream);
+    if (e.bytes < bytes || !e.ptr) {
+        // Growing frees the old buffer after a device-wide synchronisation and
+        // allocates a new one: neither is legal inside a stream capture.
+        if (capturing) {
+            *status = set_error(MGX_ERR_WORKSPACE,
+                                "library scratch must grow during a stream capture: pass a "
+                                "workspace (mgx_split_workspace_bytes) or make one uncaptured "
+                                "call of this size first");
             return nullptr;
         }
-        e.ptr = nullptr;
-        e.bytes = 0;
+        if (e.ptr) {
+            // The previous buffer may still be referenced by queued work.
+            if (hipDeviceSynchronize() != hipSuccess || hipFree(e.ptr) != hipSuccess) {
+                *status = set_error(MGX_ERR_HIP, "releasing scratch failed");
+                return nullptr;
+            }
+            e.ptr = nullptr;
+            e.bytes = 0;
+            e.used = false;
+        }
+        size_t want = bytes < 256 ? 256 : bytes;
+        if (hipMalloc(&e.ptr, want) != hipSuccess) {
+            e.ptr = nullptr;
+            *status = set_error(MGX_ERR_ALLOC, "hipMalloc of library scratch failed");
+            return nullptr;
+        }
+        e.bytes = want;
     }
-    size_t want = bytes < 256 ? 256 : bytes;
-    if (hipMalloc(&e.ptr, want) != hipSuccess) {
-        e.ptr = nullptr;
-        *status = set_error(MGX_ERR_ALLOC, "hipMalloc of library scratch failed");
-        return nullptr;
+    if (e.used && e.last_stream != stream) {
+        // The buffer was last used by work on another stream: order this
+        // call's work after it (outside a capture; a captured call must share
+        // the stream of the previous user or bring its own workspace).
+        if (capturing) {
+            *status = set_error(MGX_ERR_WORKSPACE,
+                                "library scratch was last used on another stream: a captured "
+                                "call must pass its own workspace");
+            return nullptr;
+        }
+        if (hipStreamWaitEvent(stream, e.last_use, 0) != hipSuccess) {
+            *status = set_error(MGX_ERR_HIP, "ordering library scratch across streams failed");
+            return nullptr;
+        }
     }
-    e.bytes = want;
     return e.ptr;
+}
+
+int scratch_release(int slot, void *stream_ptr) {
+    hipStream_t stream = (hipStream_t)stream_ptr;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return set_error(MGX_ERR_HIP, "hipGetDevice failed");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto &vec = g_scratch[slot];
+    if ((int)vec.size() <= dev) return MGX_OK;
+    ScratchEntry &e = vec[dev];
+    e.last_stream = stream;
+    e.used = true;
+    if (is_capturing(stream)) return MGX_OK;   // the capture orders its own nodes
+    if (!e.last_use && hipEventCreateWithFlags(&e.last_use, hipEventDisableTiming) != hipSuccess)
+        return set_error(MGX_ERR_HIP, "hipEventCreate for library scratch failed");
+    if (hipEventRecord(e.last_use, stream) != hipSuccess)
+        return set_error(MGX_ERR_HIP, "hipEventRecord for library scratch failed");
+    return MGX_OK;
 }
 
 }  // namespace mgx
@@ -255,7 +313,7 @@ extern "C" int mgx_validate_constant(const double *dC, int64_t rows, int64_t col
     MGX_ARG_CHECK(rows * cols == 0 || dC != nullptr, "C is NULL");
     int st = MGX_OK;
     ValPartial *scratch = (ValPartial *)mgx::device_scratch(
-        mgx::SCRATCH_VALIDATE, sizeof(ValPartial) * (VAL_BLOCKS + 1), &st);
+        mgx::SCRATCH_VALIDATE, sizeof(ValPartial) * (VAL_BLOCKS + 1), nullptr, &st);
     if (!scratch) return st;
     const int64_t total = rows * cols;
     int blocks = (int)((total + VAL_THREADS - 1) / VAL_THREADS);
@@ -269,6 +327,7 @@ extern "C" int mgx_validate_constant(const double *dC, int64_t rows, int64_t col
     if ((st = mgx::launch_status("validate_fold_kernel")) != MGX_OK) return st;
     ValPartial res;
     MGX_HIP_TRY(hipMemcpy(&res, scratch + VAL_BLOCKS, sizeof(res), hipMemcpyDeviceToHost));
+    if ((st = mgx::scratch_release(mgx::SCRATCH_VALIDATE, nullptr)) != MGX_OK) return st;
 
     if (rank == 0) {
         printf("\n");

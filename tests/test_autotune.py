@@ -15,15 +15,15 @@ def test_candidate_space():
     mods = [x for x in c if x.form == "mod"]
     assert len(c) == len(at.PANELS) * len(at.TILES) * (1 + len(at.SLICES)) + len(mods)
     assert at.DEFAULT in c
-    # only FP64-accurate candidates: s >= 7, and moduli with beta >= MIN_BETA at
-    # the depth of each split product (a group of whole panels)
-    assert all(x.slices >= 7 for x in c if x.form == "int8")
+    # only FP64-accurate candidates (calibrated on stress inputs): s = 8, and
+    # moduli with beta >= MIN_BETA = 52 at the depth of each split product (a
+    # group of whole panels, at most 4096 deep by default)
+    assert {x.slices for x in c if x.form == "int8"} == {8}
     assert mods and all(at.mod_beta(x.slices, at.step_k(x, 16384)) >= at.MIN_BETA for x in mods)
-    # groups of 4096 need 15 moduli (beta(14, 4096) = 48); at k = 1024 the
-    # whole depth is one group and 14 moduli keep beta = 49
     assert {x.slices for x in mods} == {15, 16}
-    assert at.mod_beta(14, 4096) == 48 and at.mod_beta(14, 1024) == 49
-    assert {x.slices for x in at.candidate_space(1024) if x.form == "mod"} == {14, 15, 16}
+    assert at.mod_beta(15, 4096) == 52 and at.mod_beta(15, 8192) == 51
+    assert at.mod_beta(14, 1024) == 49
+    assert {x.slices for x in at.candidate_space(1024) if x.form == "mod"} == {15, 16}
     small = at.candidate_space(200)     # panels wider than K are dropped
     assert {x.panel for x in small} == {128}
     assert len({x.key() for x in c}) == len(c)

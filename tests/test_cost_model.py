@@ -66,7 +66,7 @@ def test_dataset_covers_every_candidate_of_every_shape():
         keys = {(r["form"], r["slices"], r["tile"], r["panel"]) for r in rows
                 if (r["m"], r["n"], r["k"]) == (m, n, k)}
         want = {(c.form, c.slices, c.tile, c.panel) for c in at.candidate_space(k)}
-        assert keys == want, (m, n, k)
+        assert want <= keys, (m, n, k)
     assert all(r["ms"] > 0 for r in rows)
     assert model.in_dim == len(at.FEATURE_NAMES)
 

@@ -2,9 +2,10 @@
 8192x32768, B 32768x32768 resident in HBM, C_i 8192x32768), uniform[-1,1)
 inputs, through the C ABI.
 
-Size-independent checks at the full size: the FP64-accurate split forms are
-no worse than the plain FP64 MFMA product against the double-double product
-on sampled rows; the plain product is within the FP64 gate of it; and the
+Size-independent checks at the full size: the FP64-accurate split forms
+(15 moduli at beta = 50, INT8 s = 8) are no worse than the plain FP64 MFMA
+product against the double-double product on sampled rows, and INT8 s = 6, 7,
+8 gain about 8 bits per slice; the plain product is within the FP64 gate of it; and the
 modular form's result does not depend on the tile (both tiles compute the
 same integers and round once)."""
 import numpy as np
@@ -39,12 +40,19 @@ def test_share_n32768_p4_split_forms_no_worse_than_plain(mgx):
                   B.data_ptr(), C.data_ptr(), m, n, k, k, n, n, 0, None, 0, 0)
     assert np.array_equal(C[:: 1024, :: 1024].cpu().numpy(), sample)
 
-    mgx.split_gemm(A, B, C, m, n, k, k, n, n, form=mgx.FORM_INT8, slices=8)
-    slices = mgx.error_report(A, B, C, m, n, k, k, n, n, rows)
-    print(f"normwise plain {plain['normwise']:.3e} mod15 {mod['normwise']:.3e} "
-          f"int8 s=8 {slices['normwise']:.3e}", flush=True)
+    # INT8 digit slices s = 6, 7, 8 (the configuration's range): each slice
+    # adds 8 bits until FP64 rounding; s = 8 is no worse than plain
+    sl = {}
+    for s in (6, 7, 8):
+        mgx.split_gemm(A, B, C, m, n, k, k, n, n, form=mgx.FORM_INT8, slices=s)
+        sl[s] = mgx.error_report(A, B, C, m, n, k, k, n, n, rows)
+    print(f"normwise plain {plain['normwise']:.3e} mod15 {mod['normwise']:.3e} " +
+          " ".join(f"int8 s={s} {sl[s]['normwise']:.3e}" for s in sl), flush=True)
+    assert sl[7]["normwise"] < sl[6]["normwise"] / 64
+    assert sl[8]["normwise"] < sl[7]["normwise"]
     assert mod["normwise"] <= plain["normwise"]
-    assert slices["normwise"] <= plain["normwise"]
     assert mod["componentwise"] <= plain["componentwise"]
+    assert sl[8]["normwise"] <= plain["normwise"]
+    assert sl[8]["componentwise"] <= plain["componentwise"]
     del A, B, C
     torch.cuda.empty_cache()

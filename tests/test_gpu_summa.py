@@ -93,6 +93,41 @@ def test_real_panels_match_reference_order(mgx, orc):
     assert np.all(np.abs(C - ref) <= bound)
 
 
+@pytest.mark.parametrize("form", ["fp64", "mod"])
+def test_panel_width_1024(mgx, orc, form):
+    """The widest panel of the 2x2 campaign (b = 1024) with a ragged last
+    panel (k = 5000: four panels of 1024 and one of 904): per-panel reference
+    order bound (FP64) or the grouped oracle model (15 moduli), and graph,
+    stream and pageable runs equal bit for bit."""
+    m, n, k, panel = 384, 320, 5000, 1024
+    announce(f"summa b=1024 form={form}")
+    A = orc.fill(m, k, 1, seed=41)
+    B = orc.fill(k, n, 1, seed=42)
+    code, sl = (0, 8) if form == "fp64" else (mgx.FORM_INT8_MOD, 15)
+    C, st = _run(mgx, A, B, panel, pinned=True, form=code, slices=sl)
+    assert st["panels"] == 5
+    C_graph, _ = _run(mgx, A, B, panel, pinned=True, graph=True, form=code, slices=sl)
+    C_page, _ = _run(mgx, A, B, panel, pinned=False, form=code, slices=sl)
+    assert np.array_equal(C, C_graph) and np.array_equal(C, C_page)
+    if form == "fp64":
+        ref = np.zeros((m, n))
+        bound = np.zeros((m, n))
+        u = 2.0 ** -53
+        for k0 in range(0, k, panel):
+            kc = min(panel, k - k0)
+            Ap = np.ascontiguousarray(A[:, k0:k0 + kc])
+            Bp = np.ascontiguousarray(B[k0:k0 + kc])
+            ref = orc.gemm_accumulate_order(Ap, Bp, ref, kc, 0, kc)
+            absp = orc.gemm_abs_rows(Ap, Bp)
+            bound += 2 * (kc * u / (1 - kc * u)) * absp + 4 * u * (np.abs(ref) + absp)
+        assert np.all(np.abs(C - ref) <= bound)
+    else:
+        g = mgx.split_group_k(code, panel, k)
+        assert g == 4096
+        ref = _grouped(A, B, g, lambda a, b, C: orc.crt_gemm(a, b, 15, C=C))
+        assert np.array_equal(C, ref)
+
+
 def test_overlapped_equals_serialized_and_graph(mgx, orc):
     m, n, k, panel = 320, 288, 1500, 512
     announce("summa overlap vs serialize vs graph")
