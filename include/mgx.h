@@ -168,6 +168,23 @@ int mgx_split_gemm_ex(int form, int slices, int tile,
                       int accumulate, void *workspace, size_t workspace_bytes,
                       void *stream);
 
+/* No counterpart in the reference: mgx_split_gemm for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid allocation).  It calls no hipMalloc / hipFree and
+ * takes no workspace: the workspace is a library-owned static device array of
+ * MGX_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.  Calls
+ * whose mgx_split_workspace_bytes(form, slices, m, n, k) exceeds it return
+ * MGX_ERR_WORKSPACE before any launch (for example, 15 moduli fit up to
+ * 1024 x 1024 x 1024: 45 MiB).  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, other host threads) race.
+ * Otherwise as mgx_split_gemm, with the same result bit for bit. */
+#define MGX_NOALLOC_SCRATCH_BYTES (64u << 20)
+int mgx_split_gemm_noalloc(int form, int slices,
+                           const double *A, const double *B, double *C,
+                           int64_t m, int64_t n, int64_t k,
+                           int64_t lda, int64_t ldb, int64_t ldc,
+                           int accumulate, void *stream);
+
 /* Supported slice-count range for a split form (inclusive); for
  * MGX_FORM_INT8_MOD the count is the number of moduli. */
 int mgx_split_slices_range(int form, int *min_slices, int *max_slices);

@@ -21,7 +21,7 @@ from ._lib import FORM_BF16, FORM_FP64, FORM_INT8, FORM_INT8_MOD, MgxError, lib
 
 __all__ = [
     "ABMultiply", "ABMultiplyAccumulateAsync", "validate_matrix_C", "split_gemm",
-    "split_gemm_events", "split_workspace_bytes", "error_report", "summa_run", "split_group_k", "fill", "fill_exponents", "scale_rows_cols", "version", "peak",
+    "split_gemm_events", "split_gemm_noalloc", "split_workspace_bytes", "error_report", "summa_run", "split_group_k", "fill", "fill_exponents", "scale_rows_cols", "version", "peak",
     "FORM_FP64", "FORM_INT8", "FORM_INT8_MOD", "FORM_BF16", "MgxError", "lib",
 ]
 
@@ -86,6 +86,14 @@ def split_gemm(a, b, c, m, n, k, lda, ldb, ldc, *, form=FORM_INT8, slices=8, acc
     """FP64-accurate C = A.B (or C += A.B) through INT8 slice products."""
     _lib.call("mgx_split_gemm", form, slices, _ptr(a), _ptr(b), _ptr(c), m, n, k, lda, ldb, ldc,
               1 if accumulate else 0, _ptr(workspace), workspace_bytes, _stream(stream))
+
+
+def split_gemm_noalloc(a, b, c, m, n, k, lda, ldb, ldc, *, form=FORM_INT8, slices=8,
+                      accumulate=False, stream=None) -> None:
+    """split_gemm without any device allocation: the workspace is the library's
+    static per-GPU scratch (mgx_split_gemm_noalloc; sizes beyond it raise)."""
+    _lib.call("mgx_split_gemm_noalloc", form, slices, _ptr(a), _ptr(b), _ptr(c), m, n, k, lda,
+              ldb, ldc, 1 if accumulate else 0, _stream(stream))
 
 
 def split_gemm_events(a, b, c, m, n, k, lda, ldb, ldc, events, *, form=FORM_INT8, slices=8,

@@ -20,6 +20,7 @@ MGX_ERR_HIP = 2
 MGX_ERR_UNSUPPORTED = 3
 MGX_ERR_WORKSPACE = 4
 MGX_ERR_ALLOC = 5
+NOALLOC_SCRATCH_BYTES = 64 << 20   # MGX_NOALLOC_SCRATCH_BYTES
 
 FORM_FP64 = 0
 FORM_INT8 = 1
@@ -80,6 +81,8 @@ _SIGNATURES = {
     "mgx_split_gemm_events": (_int, [_int, _int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64,
                                      _i64, _int, _vp, ctypes.c_size_t, _vp,
                                      ctypes.POINTER(_vp)]),
+    "mgx_split_gemm_noalloc": (_int, [_int, _int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64,
+                                      _i64, _int, _vp]),
     "mgx_split_slices_range": (_int, [_int, ctypes.POINTER(_int), ctypes.POINTER(_int)]),
     "mgx_split_mod_beta": (_int, [_int, _i64]),
     "mgx_error_report": (_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _i64,
