@@ -100,6 +100,18 @@ int mgx_dgemm_ex(int tile, int accumulate, const double *A, const double *B, dou
                  int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc,
                  void *stream);
 
+/* The panel-fused FP64 product: C = (accumulate == 0) or C += the sum over
+ * panels p of A[:, p] . B[p, :], panels of width `panel` along k (ragged last
+ * panel), each panel's partial formed from zero and added into C in panel
+ * order -- the result of one ABMultiplyAccumulateAsync call per panel
+ * (2D-block-matrix/mmb.cu:356-358), bit for bit, in one launch that reads and
+ * writes C once.  No counterpart in the reference: it fuses the reference's
+ * per-panel launch sequence; mgx_dgemm is the case panel >= k.  `tile` is a
+ * MGX_TILE_* variant.  Asynchronous on `stream`. */
+int mgx_dgemm_panels(int tile, int accumulate, const double *A, const double *B, double *C,
+                     int64_t m, int64_t n, int64_t k, int64_t panel,
+                     int64_t lda, int64_t ldb, int64_t ldc, void *stream);
+
 /* ---- constant-result validation ----------------------------------------
  * Replaces validate_matrix_C (1D-row-block/mulmat_kernel.cu:228-326).
  * An element is invalid iff |C-expected| > abs_tol AND |C-expected|/|expected|
@@ -228,11 +240,12 @@ int mgx_error_report(const double *A, const double *B, const double *C,
  * buffers first (the reference's pack_a / pack_b).  dC is a device m x n
  * matrix that is accumulated into (zero it for C = A.B).  Synchronous: all
  * resources are created before and released after the timed loop.
- * Split forms: panels still stream one at a time, but consecutive panels
- * are gathered into a group of mgx_split_group_k(...) columns and
- * multiplied by one split product (C += A_g . B_g per group), so the
- * exponent scan, operand conversion and reconstruction are paid per group,
- * not per panel.  The FP64 form multiplies each panel as it lands. */
+ * Consecutive panels are gathered into a group of mgx_split_group_k(...)
+ * columns (whole panels; with pinned strips and asynchronous copies one A and
+ * one B copy per group) and the group is multiplied by one launch:
+ * FP64 by mgx_dgemm_panels (one partial per panel, the reference's result),
+ * split forms by one split product C += A_g . B_g, so the exponent scan,
+ * operand conversion and reconstruction are paid per group, not per panel. */
 enum {
     MGX_SUMMA_COPY_ASYNC = 0,      /* copies overlap compute (MMM, YYY, NNN codes) */
     MGX_SUMMA_COPY_HOST_SYNC = 1   /* host waits for each panel copy (CCC, XXX, WWW, SSS) */
@@ -246,8 +259,8 @@ typedef struct mgx_summa_params {
     int copy_mode;      /* MGX_SUMMA_COPY_* */
     int use_graph;      /* capture the whole panel loop in one hipGraph (pinned strips) */
     int serialize;      /* debug: synchronise after every copy and every product */
-    int64_t split_k;    /* split forms: columns per split product (whole panels);
-                           <= 0 selects MGX_SPLIT_GROUP_K */
+    int64_t split_k;    /* columns per launch (whole panels); <= 0 selects the
+                           form's default group (mgx_split_group_k) */
 } mgx_summa_params;
 
 typedef struct mgx_summa_stats {
@@ -256,14 +269,17 @@ typedef struct mgx_summa_stats {
     int host_pinned;    /* 1 if the host strips were pinned (no host packing) */
 } mgx_summa_stats;
 
-/* Default reduction columns per split product in a panel loop. */
+/* Default reduction columns per launch in a panel loop: split products, and
+ * FP64 panel groups (smaller: an FP64 group only saves the C traffic, and a
+ * streamed loop ends with the last group's product). */
 #define MGX_SPLIT_GROUP_K 4096
+#define MGX_FP64_GROUP_K 2048
 
-/* Reduction columns of each split product in a panel loop (SUMMA and the
+/* Reduction columns of each launch in a panel loop (SUMMA and the
  * autotuner's local product): the smallest whole number of panels covering
- * split_k columns (MGX_SPLIT_GROUP_K when split_k <= 0), capped at k; for
- * MGX_FORM_FP64 one panel.  The last group may be shorter.  Returns -1 for
- * non-positive panel or k, or an unknown form. */
+ * split_k columns (MGX_SPLIT_GROUP_K, or MGX_FP64_GROUP_K for MGX_FORM_FP64,
+ * when split_k <= 0), capped at k.  The last group may be shorter.  Returns
+ * -1 for non-positive panel or k, or an unknown form. */
 int64_t mgx_split_group_k(int form, int64_t panel, int64_t split_k, int64_t k);
 
 int mgx_summa_run(const mgx_summa_params *params,
@@ -281,9 +297,9 @@ int mgx_host_free(void *ptr);
  * MGX_FORM_INT8), slices (INT8), tile (MGX_TILE_*) and SUMMA panel width.
  * mgx_time_local_product runs the compute side of the SUMMA panel loop on
  * device-resident operands: C = sum over groups g of A[:, g] . B[g, :], with
- * groups of one panel (FP64) or mgx_split_group_k(form, panel, split_k, k)
- * columns (split forms); the first group overwrites C, later groups
- * accumulate.  `warmup` untimed then `reps` timed repetitions; returns the
+ * groups of mgx_split_group_k(form, panel, split_k, k) columns, each one
+ * launch (FP64: mgx_dgemm_panels, one partial per panel); the first group
+ * overwrites C, later groups accumulate.  `warmup` untimed then `reps` timed repetitions; returns the
  * mean milliseconds per full product (hipEvent timing).  workspace may be
  * NULL (library scratch). */
 typedef struct mgx_tune_config {

@@ -67,6 +67,8 @@ _SIGNATURES = {
     "mgx_dgemm_accumulate": (_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64,
                                     _i64, _int, _vp]),
     "mgx_dgemm_ex": (_int, [_int, _int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp]),
+    "mgx_dgemm_panels": (_int, [_int, _int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64,
+                                _i64, _vp]),
     "mgx_split_gemm_ex": (_int, [_int, _int, _int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64,
                                  _i64, _int, _vp, ctypes.c_size_t, _vp]),
     "mgx_time_local_product": (_int, [ctypes.POINTER(TuneConfig), _vp, _vp, _vp, _i64, _i64, _i64,

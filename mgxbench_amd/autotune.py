@@ -82,8 +82,8 @@ def mod_beta(moduli: int, k: int) -> int:
 
 
 def step_k(c: Candidate, k: int) -> int:
-    """Reduction depth of each product of the panel loop: one panel (FP64) or
-    a group of whole panels (split forms; mgx_split_group_k, default group)."""
+    """Reduction depth of each launch of the panel loop: a group of whole
+    panels (mgx_split_group_k with the form's default group)."""
     return int(_lib.lib().mgx_split_group_k(FORM_CODE[c.form], c.panel, 0, k))
 
 

@@ -1,39 +1,150 @@
-// Plain FP64 GEMM on the CDNA4 FP64 matrix cores (v_mfma_f64_16x16x4_f64).
+// Plain FP64 GEMM on the CDNA4 FP64 matrix cores (v_mfma_f64_16x16x4_f64),
+// and its panel-fused form.
 //
 // Replaces the reference's tiled kernels sharedABMultiply /
 // sharedABMultiplyAccumulate (1D-row-block/mulmat_kernel.cu:39-80, 100-142;
-// 2D-block-matrix/mulmat_kernel.cu:36-75, 94-136) and their launchers
+// 2D-block-matrix/mulmat_kernel.cu:36-75, 94-136), their launchers
 // ABMultiply / ABMultiplyAccumulateAsync (1D-row-block/mulmat_kernel.cu:86-92,
-// 148-158).  Semantics kept: row-major operands with leading dimensions,
-// zero contribution from out-of-range A columns (k_offset + kk >= k_total),
-// and the accumulate form adds one per-chunk partial product into C.
+// 148-158), and the per-panel launch sequence of the SUMMA loop
+// (2D-block-matrix/mmb.cu:239-361).  Semantics kept: row-major operands with
+// leading dimensions, out-of-range A columns contribute nothing, and every
+// panel's partial product is formed from zero and then added into C once, in
+// panel order (C += A_p . B_p per panel, as the reference's one accumulate
+// launch per panel does).
 //
-// Design (gfx950): one workgroup of 256 lanes (4 wave64s) owns a 128x128
-// block of C (64x64 for the small-tile variant); each wave owns a quarter,
-// held as 16x16 MFMA accumulators (64 doubles per lane at 128x128).  A and B k-slabs of depth 16 are
-// staged through LDS with two buffers: the global loads of slab t+1 are in
-// flight while the MFMAs of slab t run, so one barrier per slab suffices.
-// A is stored k-major in LDS so that both MFMA operands are read as 16
-// consecutive doubles per 16-lane group.  Workgroup ids are remapped so
-// that the tiles resident on one XCD (ids are dealt round-robin over the 8
-// XCDs) form a compact group of C rows, which keeps their A and B slabs in
-// that XCD's L2.
+// One kernel, dgemm_panels, does all of it: C (+)= sum_p A[:, p] . B[p, :]
+// over panels of width `panel` (ragged last panel).  A plain product is one
+// panel of depth K.  Each workgroup holds its C tile in registers for the
+// whole reduction: the running sum and the current panel's MFMA partial are
+// two register sets, and at a panel's end the partial is added into the sum
+// (one FP64 add per element) and reset, so C is read and written once per
+// launch however many panels the launch covers.
+//
+// Design (gfx950):
+//  * Tile: 128 x 128 C block per workgroup of 8 wave64s (2 x 4), each wave
+//    64 x 32 as 4 x 2 MFMA fragments; one workgroup per CU, two waves per
+//    SIMD.  Small variant: 64 x 64 with 4 waves of 32 x 32, two per CU.
+//  * Staging: a ring of NSTAGE LDS slots, each the A and B tiles of one
+//    16-deep k stage (32 KiB at 128 x 128), filled by direct global->LDS loads
+//    (global_load_lds_dwordx4: lane l writes its 16 bytes at l * 16 of a 1 KiB
+//    run, no registers involved); every stage issues one stage NSTAGE-1
+//    ahead, published with an explicit vmcnt wait plus a raw s_barrier.
+//  * A stays row-major in LDS (128-byte rows of 16 k).  16-byte chunk c of
+//    row r sits at position c ^ (r % 8): the loads pick which chunk each lane
+//    fetches, so the swizzle costs nothing.  An A fragment read
+//    (ds_read_b128) gives a lane two consecutive k of its row, which feed two
+//    consecutive MFMA steps: within a pair of steps lane group g = l / 16
+//    takes k = 8P + 2g + h for step h of pair P.  B is read with the same k
+//    assignment; the 8 lanes of each read pass touch 8 different chunk
+//    positions, so neither operand has bank conflicts.
+//  * Edges: rows of A past M and columns of B past N are clamped to valid
+//    addresses and their outputs discarded.  A stage that crosses the end of
+//    its panel, and every stage of operands that are not 16-byte aligned with
+//    even leading dimensions and N, is loaded through registers with explicit
+//    bounds (zeros past the panel end) and stored to the same LDS layout.
+//  * Tile order: persistent grid of one workgroup per CU; the tiles of the
+//    grouped raster (8 tile rows per group) are dealt out in 8 contiguous
+//    ranges, one per XCD (workgroup w runs on XCD w % 8), so the tiles in
+//    flight on an XCD share A rows and B columns in its L2.
 #include "mgx_internal.hpp"
+
+#include <mutex>
 
 namespace {
 
-constexpr int BK = 16;
-constexpr int NT = 256;
-constexpr int GROUP_M = 8;   // C tile rows per raster group
+constexpr int KS = 16;        // k depth of one LDS stage
+constexpr int NSTAGE = 4;     // LDS ring depth (stages)
+constexpr int GROUP_M = 8;    // C tile rows per raster group
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
 
-// Maps the linear workgroup id to a (tile_m, tile_n) pair: XCD-contiguous
-// renumbering followed by a grouped raster over GROUP_M tile rows.
-__device__ inline void tile_coords(int tiles_m, int tiles_n, int &tm, int &tn) {
-    const int nb = tiles_m * tiles_n;
-    int b = blockIdx.x;
-    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+// s_waitcnt with vmcnt <= N, other counters untouched (gfx9 encoding).
+template <int N>
+__device__ inline void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8) | ((N >> 4) << 14));
+}
+
+// s_waitcnt lgkmcnt(0), other counters untouched.
+__device__ inline void wait_lgkm0() {
+    __builtin_amdgcn_s_waitcnt(0xF | (0x7 << 4) | (0x0 << 8) | (0x3 << 14));
+}
+
+__device__ inline void block_barrier() {
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// 16-byte direct global->LDS load: lane l writes its bytes at lds_run + 16*l
+// (lds_run must be wave-uniform).
+__device__ inline void dma16(const double *g, char *lds_run) {
+    __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)lds_run, 16, 0,
+                                     0);
+}
+
+template <int BM_, int BN_, int WGM_, int WGN_>
+struct DShape {
+    static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_;
+    static constexpr int NW = WGM * WGN;
+    static constexpr int THREADS = 64 * NW;
+    static constexpr int WTM = BM / WGM;          // wave tile rows
+    static constexpr int WTN = BN / WGN;          // wave tile columns
+    static constexpr int FI = WTM / 16;           // MFMA fragments along rows
+    static constexpr int FJ = WTN / 16;           // ... along columns
+    static constexpr int A_BYTES = BM * KS * 8;   // A tile of one stage
+    static constexpr int B_BYTES = KS * BN * 8;   // B tile of one stage
+    static constexpr int SLOT = A_BYTES + B_BYTES;
+    static constexpr int RA = A_BYTES / 1024 / NW;   // 1 KiB A runs per wave per stage
+    static constexpr int RB = B_BYTES / 1024 / NW;   // ... B runs
+    static constexpr int LPB = BN / 2;            // lanes per B k-row (16 bytes each)
+    static constexpr int BROWS = 64 / LPB;        // B k-rows per run
+    static constexpr int LDS_BYTES = NSTAGE * SLOT;
+    static constexpr int LPS = RA + RB;           // direct loads per wave per stage
+    static_assert(RA >= 1 && RA * NW * 1024 == A_BYTES, "A runs");
+    static_assert(RB >= 1 && RB * NW * 1024 == B_BYTES, "B runs");
+    static_assert(BROWS >= 1 && BROWS * LPB == 64, "B run shape");
+    static_assert(LDS_BYTES <= 160 * 1024, "LDS budget");
+};
+using Large = DShape<128, 128, 2, 4>;
+using Small = DShape<64, 64, 2, 2>;
+
+struct PanelArgs {
+    const double *A, *B;
+    double *C;
+    int64_t M, N, K, lda, ldb, ldc;
+    int64_t panel;        // panel width; K for a plain product
+    int64_t stages;       // k stages over all panels
+    int accumulate;       // C += (else C =)
+    int dma;              // operands allow direct 16-byte loads
+    int tiles_m, tiles_n;
+};
+
+// Walks the k stages panel by panel: stage [k0, k0 + KS) of the panel that
+// ends at pend.  Past the last stage it stays there.
+struct StageCursor {
+    int64_t K, panel, pend, k0;
+    __device__ void start(int64_t K_, int64_t panel_) {
+        K = K_;
+        panel = panel_;
+        k0 = 0;
+        pend = panel < K ? panel : K;
+    }
+    __device__ bool full() const { return k0 + KS <= pend; }
+    __device__ bool last_in_panel() const { return k0 + KS >= pend; }
+    __device__ void advance() {
+        if (k0 + KS < pend) {
+            k0 += KS;
+        } else if (pend < K) {
+            k0 = pend;
+            pend = pend + panel < K ? pend + panel : K;
+        }
+    }
+};
+
+// Raster index -> (tile_m, tile_n): groups of GROUP_M tile rows, column-major
+// within a group.
+__device__ inline void raster(int b, int tiles_m, int tiles_n, int &tm, int &tn) {
     const int per_group = GROUP_M * tiles_n;
     const int g = b / per_group;
     const int first_m = g * GROUP_M;
@@ -43,153 +154,275 @@ __device__ inline void tile_coords(int tiles_m, int tiles_n, int &tm, int &tn) {
     tn = r / gsize;
 }
 
-// BM x BN block of C per workgroup, 2x2 waves of (BM/2) x (BN/2), each held as
-// (BM/32) x (BN/32) MFMA 16x16 accumulators.
-template <int BM, int BN, bool ACCUM>
-__global__ __launch_bounds__(NT, 2) void dgemm_mfma(
-    const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C,
-    int64_t M, int64_t N, int64_t K, int64_t lda, int64_t ldb, int64_t ldc) {
-    constexpr int SA = BM + 16;  // LDS row stride (doubles) of the k-major A slab
-    constexpr int SB = BN + 16;  // LDS row stride (doubles) of the B slab
-    constexpr int FI = BM / 32;  // MFMA tiles per wave along m
-    constexpr int FJ = BN / 32;  // MFMA tiles per wave along n
-    constexpr int EA = BM * BK / NT;  // A elements per lane per slab
-    constexpr int EB = BK * BN / NT;  // B elements per lane per slab
-    __shared__ double As[2][BK][SA];
-    __shared__ double Bs[2][BK][SB];
-
-    const int tiles_m = (int)((M + BM - 1) / BM);
-    const int tiles_n = (int)((N + BN - 1) / BN);
-    int tm, tn;
-    tile_coords(tiles_m, tiles_n, tm, tn);
-    const int64_t m0 = (int64_t)tm * BM;
-    const int64_t n0 = (int64_t)tn * BN;
+template <class S>
+__global__ __launch_bounds__(S::THREADS, 1) void dgemm_panels(PanelArgs a) {
+    constexpr int FI = S::FI, FJ = S::FJ, SLOT = S::SLOT;
+    __shared__ __attribute__((aligned(1024))) char lds[S::LDS_BYTES];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int wm = wave >> 1;
-    const int wn = wave & 1;
-
-    // Global->LDS load assignment: each lane takes EA consecutive k of one A
-    // row and EB consecutive columns of one B row.
-    const int a_row = tid / (BK / EA);
-    const int a_k = (tid % (BK / EA)) * EA;
-    const int b_row = tid / (BN / EB);
-    const int b_col = (tid % (BN / EB)) * EB;
-
-    const int64_t ga_row = m0 + a_row;
-    const bool a_row_ok = ga_row < M;
-    const double *a_ptr = A + (a_row_ok ? ga_row : 0) * lda;
-    const int64_t gb_col = n0 + b_col;
-
-    double ra[EA], rb[EB];
-
-    auto load_slab = [&](int64_t k0) {
-#pragma unroll
-        for (int j = 0; j < EA; ++j) {
-            const int64_t kk = k0 + a_k + j;
-            ra[j] = (a_row_ok && kk < K) ? a_ptr[kk] : 0.0;
-        }
-        const int64_t kb = k0 + b_row;
-        const bool b_row_ok = kb < K;
-        const double *b_ptr = B + (b_row_ok ? kb : 0) * ldb;
-#pragma unroll
-        for (int j = 0; j < EB; ++j) {
-            const int64_t c = gb_col + j;
-            rb[j] = (b_row_ok && c < N) ? b_ptr[c] : 0.0;
-        }
-    };
-    auto store_slab = [&](int buf) {
-#pragma unroll
-        for (int j = 0; j < EA; ++j) As[buf][a_k + j][a_row] = ra[j];
-#pragma unroll
-        for (int j = 0; j < EB; ++j) Bs[buf][b_row][b_col + j] = rb[j];
-    };
-
-    v4d acc[FI][FJ];
-#pragma unroll
-    for (int i = 0; i < FI; ++i)
-#pragma unroll
-        for (int j = 0; j < FJ; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-
-    const int nslabs = (int)((K + BK - 1) / BK);
-    const int lk = lane >> 4;   // k row within an MFMA step
-    const int lc = lane & 15;   // row (A) / column (B) within a 16-wide fragment
-
-    if (nslabs > 0) {
-        load_slab(0);
-        store_slab(0);
-        __syncthreads();
-    }
-    for (int s = 0; s < nslabs; ++s) {
-        const int buf = s & 1;
-        const bool more = (s + 1) < nslabs;
-        if (more) load_slab((int64_t)(s + 1) * BK);
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            double af[FI], bf[FJ];
-#pragma unroll
-            for (int i = 0; i < FI; ++i) af[i] = As[buf][kk + lk][wm * (BM / 2) + i * 16 + lc];
-#pragma unroll
-            for (int j = 0; j < FJ; ++j) bf[j] = Bs[buf][kk + lk][wn * (BN / 2) + j * 16 + lc];
-#pragma unroll
-            for (int i = 0; i < FI; ++i)
-#pragma unroll
-                for (int j = 0; j < FJ; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
-        }
-        if (more) store_slab(buf ^ 1);
-        __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave / S::WGN;
+    const int wn = wave % S::WGN;
+    const int64_t M = a.M, N = a.N;
+    const int tiles = a.tiles_m * a.tiles_n;
+    const int G = gridDim.x;
+    // Tiles of workgroup w: with one round of workgroups, tile w (renumbered
+    // so that each XCD takes a contiguous range); with fewer workgroups than
+    // tiles, every (G/8)-th tile of XCD w % 8's contiguous range.
+    const int w = blockIdx.x;
+    int first, stride, limit;
+    if (G >= tiles) {
+        first = ((tiles & 7) == 0) ? (w & 7) * (tiles >> 3) + (w >> 3) : w;
+        stride = tiles;
+        limit = tiles;
+    } else {
+        const int per_xcd = (tiles + 7) >> 3;
+        first = (w & 7) * per_xcd + (w >> 3);
+        stride = G >> 3;
+        limit = min(((w & 7) + 1) * per_xcd, tiles);
     }
 
-    // FP64 16x16 accumulator layout: lane l, register r holds row 4*r + l/16,
-    // column l%16 of the fragment (register r covers rows 4r..4r+3).
+    // lane-constant pieces of the load and read addresses
+    const int a_row_in_run = lane >> 3;                    // A: row within an 8-row run
+    const int a_chunk = (lane & 7) ^ (lane >> 3);          // global chunk this lane fetches
+    const int b_row_in_run = lane / S::LPB;                // B: k-row within a run
+    const int b_col = 2 * (lane % S::LPB);                 // B: first column of the lane
+    const int lc = lane & 15, lg = lane >> 4;
+
+    for (int t = first; t < limit; t += stride) {
+        int tm, tn;
+        raster(t, a.tiles_m, a.tiles_n, tm, tn);
+        const int64_t m0 = (int64_t)tm * S::BM;
+        const int64_t n0 = (int64_t)tn * S::BN;
+
+        // running sum of the C tile, and the current panel's partial
+        v4d csum[FI][FJ], acc[FI][FJ];
+        const v4d vzero = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int i = 0; i < FI; ++i) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int64_t row = m0 + wm * (BM / 2) + i * 16 + 4 * r + lk;
-            if (row >= M) continue;
-            double *c_row = C + row * ldc;
+        for (int i = 0; i < FI; ++i)
 #pragma unroll
             for (int j = 0; j < FJ; ++j) {
-                const int64_t col = n0 + wn * (BN / 2) + j * 16 + lc;
-                if (col < N) {
-                    if (ACCUM)
-                        c_row[col] = c_row[col] + acc[i][j][r];
-                    else
-                        c_row[col] = acc[i][j][r];
+                acc[i][j] = vzero;
+                csum[i][j] = vzero;
+                if (a.accumulate) {
+                    // FP64 MFMA layout: lane l, register r -> row 4r + l/16, column l%16
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int64_t row = m0 + wm * S::WTM + 16 * i + 4 * r + lg;
+                        const int64_t col = n0 + wn * S::WTN + 16 * j + lc;
+                        csum[i][j][r] = (row < M && col < N) ? a.C[row * a.ldc + col] : 0.0;
+                    }
                 }
             }
+
+        // per-lane global rows / columns of the direct loads (clamped inside
+        // the matrix; outputs of clamped rows and columns are discarded)
+        const double *a_src[S::RA];
+#pragma unroll
+        for (int j = 0; j < S::RA; ++j) {
+            const int64_t row = m0 + (wave * S::RA + j) * 8 + a_row_in_run;
+            a_src[j] = a.A + (row < M ? row : M - 1) * a.lda + 2 * a_chunk;
         }
+        int64_t b_gcol = n0 + b_col;
+        if (b_gcol > N - 2) b_gcol = N - 2 >= 0 ? N - 2 : 0;
+
+        StageCursor ld;
+        ld.start(a.K, a.panel);
+        // Issues the loads of the loader's stage into `slot`.
+        auto issue = [&](int slot) {
+            char *base = lds + slot * SLOT;
+            if (a.dma && ld.full()) {
+#pragma unroll
+                for (int j = 0; j < S::RA; ++j)
+                    dma16(a_src[j] + ld.k0, base + (wave * S::RA + j) * 1024);
+#pragma unroll
+                for (int j = 0; j < S::RB; ++j) {
+                    const int64_t krow = ld.k0 + (wave * S::RB + j) * S::BROWS + b_row_in_run;
+                    dma16(a.B + krow * a.ldb + b_gcol, base + S::A_BYTES + (wave * S::RB + j) * 1024);
+                }
+            } else {
+                // through registers, zero past the panel end or the matrix
+                v2d va[S::RA], vb[S::RB];
+#pragma unroll
+                for (int j = 0; j < S::RA; ++j) {
+                    const int64_t row = m0 + (wave * S::RA + j) * 8 + a_row_in_run;
+                    const int64_t k = ld.k0 + 2 * a_chunk;
+#pragma unroll
+                    for (int e = 0; e < 2; ++e)
+                        va[j][e] = (row < M && k + e < ld.pend) ? a.A[row * a.lda + k + e] : 0.0;
+                }
+#pragma unroll
+                for (int j = 0; j < S::RB; ++j) {
+                    const int64_t krow = ld.k0 + (wave * S::RB + j) * S::BROWS + b_row_in_run;
+                    const int64_t col = n0 + b_col;
+#pragma unroll
+                    for (int e = 0; e < 2; ++e)
+                        vb[j][e] = (krow < ld.pend && col + e < N) ? a.B[krow * a.ldb + col + e]
+                                                                   : 0.0;
+                }
+#pragma unroll
+                for (int j = 0; j < S::RA; ++j)
+                    *(v2d *)(base + (wave * S::RA + j) * 1024 + lane * 16) = va[j];
+#pragma unroll
+                for (int j = 0; j < S::RB; ++j)
+                    *(v2d *)(base + S::A_BYTES + (wave * S::RB + j) * 1024 + lane * 16) = vb[j];
+                wait_lgkm0();
+            }
+            ld.advance();
+        };
+
+        const int64_t T = a.stages;
+        if (T > 0) {
+            for (int st = 0; st < NSTAGE - 1; ++st) issue(st);
+            int pslot = NSTAGE - 1;
+            auto publish = [&]() {
+                const int freed = pslot;
+                pslot = pslot + 1 == NSTAGE ? 0 : pslot + 1;
+                wait_vmcnt<(NSTAGE - 2) * S::LPS>();
+                block_barrier();
+                issue(freed);
+            };
+            // fragments of pair P (k 8P .. 8P+7 of the stage) from `slot`
+            auto read = [&](int slot, int P, v2d (&fa)[FI], v2d (&fb)[FJ]) {
+                const char *la = lds + slot * SLOT;
+                const char *lb = la + S::A_BYTES;
+#pragma unroll
+                for (int i = 0; i < FI; ++i) {
+                    const int r = wm * S::WTM + 16 * i + lc;
+                    const int pos = (4 * P + lg) ^ (r & 7);
+                    fa[i] = *(const v2d *)(la + r * 128 + pos * 16);
+                }
+#pragma unroll
+                for (int j = 0; j < FJ; ++j) {
+                    const int col = wn * S::WTN + 16 * j + lc;
+                    const int k = 8 * P + 2 * lg;
+                    fb[j][0] = *(const double *)(lb + k * (S::BN * 8) + col * 8);
+                    fb[j][1] = *(const double *)(lb + (k + 1) * (S::BN * 8) + col * 8);
+                }
+            };
+            auto mfma = [&](const v2d (&fa)[FI], const v2d (&fb)[FJ]) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int i = 0; i < FI; ++i)
+#pragma unroll
+                        for (int j = 0; j < FJ; ++j)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i][h], fb[j][h],
+                                                                             acc[i][j], 0, 0, 0);
+            };
+            StageCursor cs;
+            cs.start(a.K, a.panel);
+            v2d a0[FI], b0[FJ], a1[FI], b1[FJ];
+            publish();
+            read(pslot, 0, a0, b0);
+            for (int64_t s = 0; s < T; ++s) {
+                read(pslot, 1, a1, b1);
+                __builtin_amdgcn_sched_barrier(0);
+                mfma(a0, b0);
+                __builtin_amdgcn_sched_barrier(0);
+                wait_lgkm0();   // this stage's fragments are in registers: its slot is free
+                if (s + 1 < T) {
+                    publish();
+                    read(pslot, 0, a0, b0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                mfma(a1, b1);
+                __builtin_amdgcn_sched_barrier(0);
+                if (cs.last_in_panel()) {
+                    // the panel's partial joins the running sum: C += A_p . B_p
+#pragma unroll
+                    for (int i = 0; i < FI; ++i)
+#pragma unroll
+                        for (int j = 0; j < FJ; ++j) {
+                            csum[i][j] += acc[i][j];
+                            acc[i][j] = vzero;
+                        }
+                }
+                cs.advance();
+            }
+        }
+
+#pragma unroll
+        for (int i = 0; i < FI; ++i)
+#pragma unroll
+            for (int j = 0; j < FJ; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t row = m0 + wm * S::WTM + 16 * i + 4 * r + lg;
+                    const int64_t col = n0 + wn * S::WTN + 16 * j + lc;
+                    if (row < M && col < N) a.C[row * a.ldc + col] = csum[i][j][r];
+                }
+        // loads past the last stage and the C stores retire, and every wave
+        // is done with the ring, before the next tile reuses it
+        wait_vmcnt<0>();
+        block_barrier();
     }
 }
 
-template <int BM, int BN>
-int launch_tile(bool accumulate, const double *A, const double *B, double *C, int64_t m,
-                int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc, hipStream_t st) {
-    const int64_t tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN);
-    MGX_ARG_CHECK(tiles <= 0x7fffffffll, "matrix too large for one launch");
-    dim3 grid((unsigned)tiles), block(NT);
-    if (accumulate)
-        hipLaunchKernelGGL((dgemm_mfma<BM, BN, true>), grid, block, 0, st, A, B, C, m, n, k, lda,
-                           ldb, ldc);
-    else
-        hipLaunchKernelGGL((dgemm_mfma<BM, BN, false>), grid, block, 0, st, A, B, C, m, n, k, lda,
-                           ldb, ldc);
-    return mgx::launch_status("dgemm_mfma");
+int cu_count() {
+    static std::mutex mu;
+    static int cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!cached[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n <= 0)
+            n = 256;
+        cached[dev] = n;
+    }
+    return cached[dev];
+}
+
+template <class S>
+int launch_panels(const double *A, const double *B, double *C, int64_t m, int64_t n, int64_t k,
+                  int64_t panel, int64_t lda, int64_t ldb, int64_t ldc, bool accumulate,
+                  hipStream_t st) {
+    const int64_t tm = (m + S::BM - 1) / S::BM, tn = (n + S::BN - 1) / S::BN;
+    MGX_ARG_CHECK(tm * tn <= 0x7fffffffll && tm <= 0x7fffffff && tn <= 0x7fffffff,
+                  "matrix too large for one launch");
+    PanelArgs a;
+    a.A = A;
+    a.B = B;
+    a.C = C;
+    a.M = m;
+    a.N = n;
+    a.K = k;
+    a.lda = lda;
+    a.ldb = ldb;
+    a.ldc = ldc;
+    a.panel = panel < k ? panel : (k > 0 ? k : 1);
+    // k stages: whole panels of ceil(panel / KS) stages, then the ragged one
+    const int64_t spp = (a.panel + KS - 1) / KS;
+    a.stages = k > 0 ? (k / a.panel) * spp + ((k % a.panel) + KS - 1) / KS : 0;
+    a.accumulate = accumulate ? 1 : 0;
+    // direct loads need 16-byte aligned A and B rows and stage starts: even
+    // leading dimensions, even N (the last B pair of a row stays in the row)
+    // and even panel starts
+    a.dma = (((uintptr_t)A | (uintptr_t)B) & 15) == 0 && (lda & 1) == 0 && (ldb & 1) == 0 &&
+            (n & 1) == 0 && ((a.panel & 1) == 0 || a.panel >= k);
+    a.tiles_m = (int)tm;
+    a.tiles_n = (int)tn;
+    const int64_t tiles = tm * tn;
+    // one round of workgroups when it fits the CUs, else a persistent grid
+    const int per_cu = (S::LDS_BYTES > 80 * 1024) ? 1 : 2;
+    const int64_t slots = (int64_t)cu_count() * per_cu;
+    const int64_t grid = tiles <= slots ? tiles : (slots / 8) * 8;
+    hipLaunchKernelGGL((dgemm_panels<S>), dim3((unsigned)grid), dim3(S::THREADS), 0, st, a);
+    return mgx::launch_status("dgemm_panels");
 }
 
 int launch_dgemm(int tile, bool accumulate, const double *A, const double *B, double *C,
-                 int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc,
-                 hipStream_t st) {
+                 int64_t m, int64_t n, int64_t k, int64_t panel, int64_t lda, int64_t ldb,
+                 int64_t ldc, hipStream_t st) {
     if (tile == MGX_TILE_DEFAULT) tile = MGX_TILE_LARGE;
     switch (tile) {
         case MGX_TILE_LARGE:
-            return launch_tile<128, 128>(accumulate, A, B, C, m, n, k, lda, ldb, ldc, st);
+            return launch_panels<Large>(A, B, C, m, n, k, panel, lda, ldb, ldc, accumulate, st);
         case MGX_TILE_SMALL:
-            return launch_tile<64, 64>(accumulate, A, B, C, m, n, k, lda, ldb, ldc, st);
+            return launch_panels<Small>(A, B, C, m, n, k, panel, lda, ldb, ldc, accumulate, st);
         default:
             return mgx::set_error(MGX_ERR_UNSUPPORTED, "unknown tile");
     }
@@ -210,7 +443,7 @@ static int dgemm_entry(int tile, const double *A, const double *B, double *C, in
         MGX_ARG_CHECK(A != nullptr && B != nullptr, "A or B is NULL");
         MGX_ARG_CHECK(lda >= k && ldb >= n, "lda < k or ldb < n");
     }
-    return launch_dgemm(tile, false, A, B, C, m, n, k, lda, ldb, ldc, (hipStream_t)stream);
+    return launch_dgemm(tile, false, A, B, C, m, n, k, k, lda, ldb, ldc, (hipStream_t)stream);
 }
 
 static int dgemm_accumulate_entry(int tile, const double *A, const double *B_chunk, double *C,
@@ -233,8 +466,8 @@ static int dgemm_accumulate_entry(int tile, const double *A, const double *B_chu
         MGX_ARG_CHECK(lda >= k_offset + k_eff, "lda too small for k_offset + k_chunk");
         MGX_ARG_CHECK(ldb >= n, "ldb < n");
     }
-    return launch_dgemm(tile, true, k_eff > 0 ? A + k_offset : A, B_chunk, C, m, n, k_eff, lda,
-                        ldb, ldc, (hipStream_t)stream);
+    return launch_dgemm(tile, true, k_eff > 0 ? A + k_offset : A, B_chunk, C, m, n, k_eff, k_eff,
+                        lda, ldb, ldc, (hipStream_t)stream);
 }
 
 extern "C" int mgx_dgemm(const double *A, const double *B, double *C, int64_t m, int64_t n,
@@ -256,4 +489,21 @@ extern "C" int mgx_dgemm_ex(int tile, int accumulate, const double *A, const dou
     if (accumulate)
         return dgemm_accumulate_entry(tile, A, B, C, m, n, k, 0, k, lda, ldb, ldc, 32, stream);
     return dgemm_entry(tile, A, B, C, m, n, k, lda, ldb, ldc, 32, stream);
+}
+
+extern "C" int mgx_dgemm_panels(int tile, int accumulate, const double *A, const double *B,
+                                double *C, int64_t m, int64_t n, int64_t k, int64_t panel,
+                                int64_t lda, int64_t ldb, int64_t ldc, void *stream) {
+    mgx::clear_error();
+    MGX_ARG_CHECK(m >= 0 && n >= 0 && k >= 0, "negative matrix dimension");
+    MGX_ARG_CHECK(panel > 0, "panel width must be positive");
+    if (m == 0 || n == 0) return MGX_OK;
+    MGX_ARG_CHECK(C != nullptr, "C is NULL");
+    MGX_ARG_CHECK(ldc >= n, "ldc < n");
+    if (k > 0) {
+        MGX_ARG_CHECK(A != nullptr && B != nullptr, "A or B is NULL");
+        MGX_ARG_CHECK(lda >= k && ldb >= n, "lda < k or ldb < n");
+    }
+    return launch_dgemm(tile, accumulate != 0, A, B, C, m, n, k, panel, lda, ldb, ldc,
+                        (hipStream_t)stream);
 }

@@ -9,13 +9,15 @@
 // per-buffer "panel ready" event and runs the accumulate product; a
 // "compute done" event guards reuse of each of the two device buffers.
 //
-// Split forms gather consecutive panels into a group buffer (whole panels
-// covering mgx_split_group_k columns) and run one split product per group:
-// the copies overlap the previous group's product, while the exponent scan,
-// operand conversion and (modular form) reconstruction of the whole C are
-// paid once per group.  With pinned strips and asynchronous copies a group
-// arrives as one A copy and one B copy; otherwise panel by panel, the
-// reference's granularity.
+// Consecutive panels are gathered into a group buffer (whole panels covering
+// mgx_split_group_k columns) and each group is one launch: FP64 runs the
+// panel-fused kernel (mgx_dgemm_panels: one partial per panel added into the
+// register-held C tile, so C is read and written once per group); split
+// forms run one split product per group, so the exponent scan, operand
+// conversion and (modular form) reconstruction of the whole C are paid once
+// per group.  The copies overlap the previous group's product.  With pinned
+// strips and asynchronous copies a group arrives as one A copy and one B
+// copy; otherwise panel by panel, the reference's granularity.
 //
 // MI355X-native differences from the reference's loop:
 //  * With pinned host strips the A panel is gathered by the DMA engine
@@ -88,8 +90,8 @@ extern "C" int64_t mgx_split_group_k(int form, int64_t panel, int64_t split_k, i
         form != MGX_FORM_BF16)
         return -1;
     const int64_t p = panel < k ? panel : k;
-    if (form == MGX_FORM_FP64) return p;
-    const int64_t want = split_k > 0 ? split_k : MGX_SPLIT_GROUP_K;
+    const int64_t want =
+        split_k > 0 ? split_k : (form == MGX_FORM_FP64 ? MGX_FP64_GROUP_K : MGX_SPLIT_GROUP_K);
     const int64_t panels = (want + p - 1) / p;
     return panels * p < k ? panels * p : k;
 }
@@ -204,8 +206,9 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
             MGX_HIP_TRY(hipEventRecord(R.ready[cur], R.transfer));
             MGX_HIP_TRY(hipStreamWaitEvent(R.compute, R.ready[cur], 0));
             if (P->form == MGX_FORM_FP64) {
-                rc = mgx_dgemm_accumulate(R.dA[cur], R.dB[cur], dC, m, n, gk, 0, gk, gk, n, ldc,
-                                          32, R.compute);
+                // one launch per group, one partial per panel inside it
+                rc = mgx_dgemm_panels(MGX_TILE_DEFAULT, 1, R.dA[cur], R.dB[cur], dC, m, n, gk,
+                                      panel, gk, n, ldc, R.compute);
             } else {
                 rc = mgx_split_gemm(P->form, P->slices, R.dA[cur], R.dB[cur], dC, m, n, gk,
                                     gk, n, ldc, 1, R.ws, R.ws_bytes, R.compute);

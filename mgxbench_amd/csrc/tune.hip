@@ -8,15 +8,16 @@ namespace {
 int run_once(const mgx_tune_config *c, const double *A, const double *B, double *C, int64_t m,
              int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc, void *ws,
              size_t ws_bytes, hipStream_t st) {
-    // one panel per FP64 product, whole-panel groups per split product (the
-    // SUMMA loop's compute side)
-    const int64_t panel = mgx_split_group_k(c->form, c->panel, c->split_k, k);
-    for (int64_t k0 = 0; k0 < k; k0 += panel) {
-        const int64_t kc = (k - k0) < panel ? (k - k0) : panel;
+    // whole-panel groups, one launch each (the SUMMA loop's compute side):
+    // FP64 groups run the panel-fused kernel, one partial per panel
+    const int64_t group = mgx_split_group_k(c->form, c->panel, c->split_k, k);
+    for (int64_t k0 = 0; k0 < k; k0 += group) {
+        const int64_t kc = (k - k0) < group ? (k - k0) : group;
         const int acc = k0 > 0 ? 1 : 0;
         int rc;
         if (c->form == MGX_FORM_FP64)
-            rc = mgx_dgemm_ex(c->tile, acc, A + k0, B + k0 * ldb, C, m, n, kc, lda, ldb, ldc, st);
+            rc = mgx_dgemm_panels(c->tile, acc, A + k0, B + k0 * ldb, C, m, n, kc, c->panel, lda,
+                                  ldb, ldc, st);
         else
             rc = mgx_split_gemm_ex(c->form, c->slices, c->tile, A + k0, B + k0 * ldb, C, m, n, kc,
                                    lda, ldb, ldc, acc, ws, ws_bytes, st);

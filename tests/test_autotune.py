@@ -31,7 +31,9 @@ def test_candidate_space():
 
 def test_step_depth_is_one_panel_or_a_whole_panel_group():
     L = _lib.TILE_LARGE
-    assert at.step_k(at.Candidate("fp64", 0, L, 256), 16384) == 256
+    # FP64 groups of whole panels covering 2048 columns, one fused launch each
+    assert at.step_k(at.Candidate("fp64", 0, L, 256), 16384) == 2048
+    assert at.step_k(at.Candidate("fp64", 0, L, 1000), 16384) == 3000
     assert at.step_k(at.Candidate("fp64", 0, L, 256), 100) == 100
     assert at.step_k(at.Candidate("mod", 15, L, 256), 16384) == 4096
     assert at.step_k(at.Candidate("int8", 7, L, 1000), 16384) == 5000    # whole panels

@@ -174,7 +174,8 @@ def test_split_forms_multiply_whole_panel_groups(mgx, orc, form_name, split_k, g
     announce(f"summa {form_name} split_k={split_k}")
     form = {"int8": mgx.FORM_INT8, "mod": mgx.FORM_INT8_MOD, "bf16": mgx.FORM_BF16}[form_name]
     assert mgx.split_group_k(form, panel, k, split_k) == group
-    assert mgx.split_group_k(mgx.FORM_FP64, panel, k, split_k) == panel
+    # FP64 groups whole panels by the same rule (default 2048 columns)
+    assert mgx.split_group_k(mgx.FORM_FP64, panel, k, split_k) == group
     A = orc.fill(m, k, 1, seed=41)
     B = orc.fill(k, n, 1, seed=42)
     if form_name in ("int8", "bf16"):
