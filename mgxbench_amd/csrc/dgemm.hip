@@ -49,11 +49,10 @@
 #include "mgx_internal.hpp"
 
 #include <mutex>
+#include <type_traits>
 
 namespace {
 
-constexpr int KS = 16;        // k depth of one LDS stage
-constexpr int NSTAGE = 4;     // LDS ring depth (stages)
 constexpr int GROUP_M = 8;    // C tile rows per raster group
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -83,15 +82,29 @@ __device__ inline void dma16(const double *g, char *lds_run) {
                                      0);
 }
 
-template <int BM_, int BN_, int WGM_, int WGN_>
+// A pointer the caller knows to be wave-uniform, moved to scalar registers
+// (lets a load use the scalar-base + 32-bit vector-offset address form).
+__device__ inline const char *uniform_ptr(const char *p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (const char *)(((uint64_t)hi << 32) | lo);
+}
+
+// BM x BN C tile per workgroup of WGM x WGN waves; k stages of KS in a ring
+// of NSTAGE LDS slots.
+template <int BM_, int BN_, int WGM_, int WGN_, int KS_, int NSTAGE_>
 struct DShape {
-    static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_;
+    static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_, KS = KS_, NSTAGE = NSTAGE_;
     static constexpr int NW = WGM * WGN;
     static constexpr int THREADS = 64 * NW;
     static constexpr int WTM = BM / WGM;          // wave tile rows
     static constexpr int WTN = BN / WGN;          // wave tile columns
     static constexpr int FI = WTM / 16;           // MFMA fragments along rows
     static constexpr int FJ = WTN / 16;           // ... along columns
+    static constexpr int NP = KS / 8;             // pairs of MFMA steps per stage
+    static constexpr int CPR = KS / 2;            // 16-byte chunks per A row of a stage
+    static constexpr int RPR = 64 / CPR;          // A rows per 1 KiB run
     static constexpr int A_BYTES = BM * KS * 8;   // A tile of one stage
     static constexpr int B_BYTES = KS * BN * 8;   // B tile of one stage
     static constexpr int SLOT = A_BYTES + B_BYTES;
@@ -101,13 +114,33 @@ struct DShape {
     static constexpr int BROWS = 64 / LPB;        // B k-rows per run
     static constexpr int LDS_BYTES = NSTAGE * SLOT;
     static constexpr int LPS = RA + RB;           // direct loads per wave per stage
+    static_assert(KS % 16 == 0 && CPR >= 8 && RPR * CPR == 64, "stage depth");
     static_assert(RA >= 1 && RA * NW * 1024 == A_BYTES, "A runs");
     static_assert(RB >= 1 && RB * NW * 1024 == B_BYTES, "B runs");
     static_assert(BROWS >= 1 && BROWS * LPB == 64, "B run shape");
     static_assert(LDS_BYTES <= 160 * 1024, "LDS budget");
+    static_assert((NSTAGE - 2) * LPS < 64 && NSTAGE >= 2, "ring depth");
 };
-using Large = DShape<128, 128, 2, 4>;
-using Small = DShape<64, 64, 2, 2>;
+#ifndef MGX_DG_KS
+#define MGX_DG_KS 16
+#endif
+#ifndef MGX_DG_NSTAGE
+#define MGX_DG_NSTAGE 4
+#endif
+#ifndef MGX_DG_MONO
+#define MGX_DG_MONO 2
+#endif
+#ifndef MGX_DG_SPLIT
+#define MGX_DG_SPLIT 2
+#endif
+#ifndef MGX_DG_LEAD
+#define MGX_DG_LEAD 2
+#endif
+// panel-fused tiles (running sum + partial in registers)
+using Large = DShape<128, 128, 2, 4, MGX_DG_KS, MGX_DG_NSTAGE>;
+using Small = DShape<64, 64, 2, 2, 16, 4>;
+// one-panel tile (the partial is the result): 256 x 128, 8 waves of 64 x 64
+using Wide = DShape<256, 128, 4, 2, 16, 3>;
 
 struct PanelArgs {
     const double *A, *B;
@@ -122,9 +155,10 @@ struct PanelArgs {
 
 // Walks the k stages panel by panel: stage [k0, k0 + KS) of the panel that
 // ends at pend.  Past the last stage it stays there.
+template <int KS>
 struct StageCursor {
-    int64_t K, panel, pend, k0;
-    __device__ void start(int64_t K_, int64_t panel_) {
+    int K, panel, pend, k0;
+    __device__ void start(int K_, int panel_) {
         K = K_;
         panel = panel_;
         k0 = 0;
@@ -154,9 +188,13 @@ __device__ inline void raster(int b, int tiles_m, int tiles_n, int &tm, int &tn)
     tn = r / gsize;
 }
 
-template <class S>
+// FUSED: the running sum of the C tile and the current panel's partial are
+// two register sets (any number of panels).  Otherwise the launch is one
+// panel and the partial itself is added to (or written as) C at the end,
+// which frees the registers for a larger tile; the result is the same.
+template <class S, bool FUSED>
 __global__ __launch_bounds__(S::THREADS, 1) void dgemm_panels(PanelArgs a) {
-    constexpr int FI = S::FI, FJ = S::FJ, SLOT = S::SLOT;
+    constexpr int FI = S::FI, FJ = S::FJ, SLOT = S::SLOT, NSTAGE = S::NSTAGE, KS = S::KS;
     __shared__ __attribute__((aligned(1024))) char lds[S::LDS_BYTES];
 
     const int tid = threadIdx.x;
@@ -184,11 +222,12 @@ __global__ __launch_bounds__(S::THREADS, 1) void dgemm_panels(PanelArgs a) {
     }
 
     // lane-constant pieces of the load and read addresses
-    const int a_row_in_run = lane >> 3;                    // A: row within an 8-row run
-    const int a_chunk = (lane & 7) ^ (lane >> 3);          // global chunk this lane fetches
+    const int a_row_in_run = lane / S::CPR;                // A: row within a run
+    const int a_pos = lane % S::CPR;                       // A: LDS chunk position of the lane
     const int b_row_in_run = lane / S::LPB;                // B: k-row within a run
     const int b_col = 2 * (lane % S::LPB);                 // B: first column of the lane
     const int lc = lane & 15, lg = lane >> 4;
+    const v4d vzero = {0.0, 0.0, 0.0, 0.0};
 
     for (int t = first; t < limit; t += stride) {
         int tm, tn;
@@ -196,65 +235,85 @@ __global__ __launch_bounds__(S::THREADS, 1) void dgemm_panels(PanelArgs a) {
         const int64_t m0 = (int64_t)tm * S::BM;
         const int64_t n0 = (int64_t)tn * S::BN;
 
-        // running sum of the C tile, and the current panel's partial
-        v4d csum[FI][FJ], acc[FI][FJ];
-        const v4d vzero = {0.0, 0.0, 0.0, 0.0};
+        // running sum of the C tile (FUSED), and the current panel's partial
+        v4d csum[FUSED ? FI : 1][FUSED ? FJ : 1], acc[FI][FJ];
 #pragma unroll
         for (int i = 0; i < FI; ++i)
 #pragma unroll
             for (int j = 0; j < FJ; ++j) {
                 acc[i][j] = vzero;
-                csum[i][j] = vzero;
-                if (a.accumulate) {
-                    // FP64 MFMA layout: lane l, register r -> row 4r + l/16, column l%16
+                if constexpr (FUSED) {
+                    csum[i][j] = vzero;
+                    if (a.accumulate) {
+                        // FP64 MFMA layout: lane l, register r -> row 4r + l/16, column l%16
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int64_t row = m0 + wm * S::WTM + 16 * i + 4 * r + lg;
-                        const int64_t col = n0 + wn * S::WTN + 16 * j + lc;
-                        csum[i][j][r] = (row < M && col < N) ? a.C[row * a.ldc + col] : 0.0;
+                        for (int r = 0; r < 4; ++r) {
+                            const int64_t row = m0 + wm * S::WTM + 16 * i + 4 * r + lg;
+                            const int64_t col = n0 + wn * S::WTN + 16 * j + lc;
+                            csum[i][j][r] = (row < M && col < N) ? a.C[row * a.ldc + col] : 0.0;
+                        }
                     }
                 }
             }
 
         // per-lane global rows / columns of the direct loads (clamped inside
-        // the matrix; outputs of clamped rows and columns are discarded)
+        // the matrix; outputs of clamped rows and columns are discarded).
+        // Chunk c of A row r lands at LDS position c ^ (r % 8): the lane at
+        // position p fetches chunk p ^ (r % 8).
         const double *a_src[S::RA];
+        int a_chunk[S::RA];
+        uint32_t a_off[S::RA];    // byte offsets from the tile's A rows (direct loads)
 #pragma unroll
         for (int j = 0; j < S::RA; ++j) {
-            const int64_t row = m0 + (wave * S::RA + j) * 8 + a_row_in_run;
-            a_src[j] = a.A + (row < M ? row : M - 1) * a.lda + 2 * a_chunk;
+            const int rr = (wave * S::RA + j) * S::RPR + a_row_in_run;
+            a_chunk[j] = a_pos ^ (rr & 7);
+            const int64_t row = m0 + rr;
+            a_src[j] = a.A + row * a.lda + 2 * a_chunk[j];
+            const int64_t crow = (row < M ? row : M - 1) - m0;
+            a_off[j] = (uint32_t)((crow * a.lda + 2 * a_chunk[j]) * 8);
         }
         int64_t b_gcol = n0 + b_col;
         if (b_gcol > N - 2) b_gcol = N - 2 >= 0 ? N - 2 : 0;
+        uint32_t b_off[S::RB];    // byte offsets from the stage's B rows at column n0
+#pragma unroll
+        for (int j = 0; j < S::RB; ++j) {
+            const int krow = (wave * S::RB + j) * S::BROWS + b_row_in_run;
+            b_off[j] = (uint32_t)((krow * a.ldb + (b_gcol - n0)) * 8);
+        }
+        const char *a_tile = (const char *)(a.A + m0 * a.lda);
+        const char *b_tile = (const char *)(a.B + n0);
 
-        StageCursor ld;
-        ld.start(a.K, a.panel);
+        StageCursor<KS> ld;
+        ld.start((int)a.K, (int)a.panel);
         // Issues the loads of the loader's stage into `slot`.
         auto issue = [&](int slot) {
             char *base = lds + slot * SLOT;
             if (a.dma && ld.full()) {
+                // wave-uniform stage bases plus per-lane 32-bit offsets
+                const char *ab = uniform_ptr(a_tile + (int64_t)ld.k0 * 8);
+                const char *bb = uniform_ptr(b_tile + (int64_t)ld.k0 * a.ldb * 8);
 #pragma unroll
                 for (int j = 0; j < S::RA; ++j)
-                    dma16(a_src[j] + ld.k0, base + (wave * S::RA + j) * 1024);
+                    dma16((const double *)(ab + a_off[j]), base + (wave * S::RA + j) * 1024);
 #pragma unroll
-                for (int j = 0; j < S::RB; ++j) {
-                    const int64_t krow = ld.k0 + (wave * S::RB + j) * S::BROWS + b_row_in_run;
-                    dma16(a.B + krow * a.ldb + b_gcol, base + S::A_BYTES + (wave * S::RB + j) * 1024);
-                }
+                for (int j = 0; j < S::RB; ++j)
+                    dma16((const double *)(bb + b_off[j]),
+                          base + S::A_BYTES + (wave * S::RB + j) * 1024);
             } else {
                 // through registers, zero past the panel end or the matrix
                 v2d va[S::RA], vb[S::RB];
 #pragma unroll
                 for (int j = 0; j < S::RA; ++j) {
-                    const int64_t row = m0 + (wave * S::RA + j) * 8 + a_row_in_run;
-                    const int64_t k = ld.k0 + 2 * a_chunk;
+                    const int64_t row = m0 + (wave * S::RA + j) * S::RPR + a_row_in_run;
+                    const int k = ld.k0 + 2 * a_chunk[j];
 #pragma unroll
                     for (int e = 0; e < 2; ++e)
-                        va[j][e] = (row < M && k + e < ld.pend) ? a.A[row * a.lda + k + e] : 0.0;
+                        va[j][e] = (row < M && k + e < ld.pend) ? a_src[j][ld.k0 + e] : 0.0;
                 }
 #pragma unroll
                 for (int j = 0; j < S::RB; ++j) {
-                    const int64_t krow = ld.k0 + (wave * S::RB + j) * S::BROWS + b_row_in_run;
+                    const int64_t krow = (int64_t)ld.k0 + (wave * S::RB + j) * S::BROWS +
+                                         b_row_in_run;
                     const int64_t col = n0 + b_col;
 #pragma unroll
                     for (int e = 0; e < 2; ++e)
@@ -276,70 +335,105 @@ __global__ __launch_bounds__(S::THREADS, 1) void dgemm_panels(PanelArgs a) {
         if (T > 0) {
             for (int st = 0; st < NSTAGE - 1; ++st) issue(st);
             int pslot = NSTAGE - 1;
-            auto publish = [&]() {
-                const int freed = pslot;
-                pslot = pslot + 1 == NSTAGE ? 0 : pslot + 1;
+            // the next stage's loads are in LDS once at most the NSTAGE-2
+            // younger stages are outstanding; after the barrier every wave's
+            // are, and every wave is done reading the stage before it, whose
+            // slot then takes the stage NSTAGE-1 ahead
+            auto publish_wait = [&]() {
                 wait_vmcnt<(NSTAGE - 2) * S::LPS>();
                 block_barrier();
+            };
+            auto publish_issue = [&]() {
+                const int freed = pslot;
+                pslot = pslot + 1 == NSTAGE ? 0 : pslot + 1;
                 issue(freed);
             };
-            // fragments of pair P (k 8P .. 8P+7 of the stage) from `slot`
-            auto read = [&](int slot, int P, v2d (&fa)[FI], v2d (&fb)[FJ]) {
+            // fragments of pair P (k 8P .. 8P+7 of the stage) from `slot`: A as
+            // one 16-byte read per fragment (k and k+1 of the lane's row), B
+            // as single doubles (k row and k+1 row), kept apart so the reads
+            // can be merged freely
+            auto read = [&](int slot, int P, v2d (&fa)[FI], double (&fb)[2][FJ]) {
                 const char *la = lds + slot * SLOT;
                 const char *lb = la + S::A_BYTES;
 #pragma unroll
                 for (int i = 0; i < FI; ++i) {
                     const int r = wm * S::WTM + 16 * i + lc;
                     const int pos = (4 * P + lg) ^ (r & 7);
-                    fa[i] = *(const v2d *)(la + r * 128 + pos * 16);
+                    fa[i] = *(const v2d *)(la + r * (KS * 8) + pos * 16);
                 }
 #pragma unroll
                 for (int j = 0; j < FJ; ++j) {
                     const int col = wn * S::WTN + 16 * j + lc;
                     const int k = 8 * P + 2 * lg;
-                    fb[j][0] = *(const double *)(lb + k * (S::BN * 8) + col * 8);
-                    fb[j][1] = *(const double *)(lb + (k + 1) * (S::BN * 8) + col * 8);
+                    fb[0][j] = *(const double *)(lb + k * (S::BN * 8) + col * 8);
+                    fb[1][j] = *(const double *)(lb + (k + 1) * (S::BN * 8) + col * 8);
                 }
             };
-            auto mfma = [&](const v2d (&fa)[FI], const v2d (&fb)[FJ]) {
+            // MFMAs [q0, q1) of the pair's 2 * FI * FJ, in (h, i, j) order
+            auto mfma = [&](const v2d (&fa)[FI], const double (&fb)[2][FJ], auto q0, auto q1) {
+                constexpr int Q0 = decltype(q0)::value, Q1 = decltype(q1)::value;
 #pragma unroll
-                for (int h = 0; h < 2; ++h)
-#pragma unroll
-                    for (int i = 0; i < FI; ++i)
-#pragma unroll
-                        for (int j = 0; j < FJ; ++j)
-                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i][h], fb[j][h],
-                                                                             acc[i][j], 0, 0, 0);
+                for (int q = Q0; q < Q1; ++q) {
+                    const int h = q / (FI * FJ), i = (q / FJ) % FI, j = q % FJ;
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i][h], fb[h][j],
+                                                                     acc[i][j], 0, 0, 0);
+                }
             };
-            StageCursor cs;
-            cs.start(a.K, a.panel);
-            v2d a0[FI], b0[FJ], a1[FI], b1[FJ];
-            publish();
-            read(pslot, 0, a0, b0);
+            using Q0 = std::integral_constant<int, 0>;
+            using QH = std::integral_constant<int, MGX_DG_SPLIT>;
+            using QA = std::integral_constant<int, MGX_DG_LEAD>;
+            using QE = std::integral_constant<int, 2 * FI * FJ>;
+            StageCursor<KS> cs;
+            cs.start((int)a.K, (int)a.panel);
+            v2d fa[2][FI];
+            double fb[2][2][FJ];
+            publish_wait();
+            publish_issue();
+            read(pslot, 0, fa[0], fb[0]);
+            wait_lgkm0();
             for (int64_t s = 0; s < T; ++s) {
-                read(pslot, 1, a1, b1);
-                __builtin_amdgcn_sched_barrier(0);
-                mfma(a0, b0);
-                __builtin_amdgcn_sched_barrier(0);
-                wait_lgkm0();   // this stage's fragments are in registers: its slot is free
-                if (s + 1 < T) {
-                    publish();
-                    read(pslot, 0, a0, b0);
-                }
-                __builtin_amdgcn_sched_barrier(0);
-                mfma(a1, b1);
-                __builtin_amdgcn_sched_barrier(0);
-                if (cs.last_in_panel()) {
-                    // the panel's partial joins the running sum: C += A_p . B_p
 #pragma unroll
-                    for (int i = 0; i < FI; ++i)
-#pragma unroll
-                        for (int j = 0; j < FJ; ++j) {
-                            csum[i][j] += acc[i][j];
-                            acc[i][j] = vzero;
-                        }
+                for (int P = 0; P < S::NP; ++P) {
+                    const int cur = P & 1;
+                    if (P + 1 < S::NP) {
+                        // a few MFMAs first, so no fragment read issued just
+                        // before them stands in front of the matrix core
+                        mfma(fa[cur], fb[cur], Q0{}, QA{});
+                        __builtin_amdgcn_sched_barrier(0);
+                        read(pslot, P + 1, fa[cur ^ 1], fb[cur ^ 1]);
+                        __builtin_amdgcn_sched_barrier(0);
+                        mfma(fa[cur], fb[cur], QA{}, QE{});
+                        __builtin_amdgcn_sched_barrier(0);
+                    } else {
+                        wait_lgkm0();   // the stage's fragments are in registers
+                        const bool more = s + 1 < T;
+                        if (more) publish_wait();
+                        __builtin_amdgcn_sched_barrier(0);
+                        mfma(fa[cur], fb[cur], Q0{}, QH{});
+                        __builtin_amdgcn_sched_barrier(0);
+                        if (more) publish_issue();
+                        // unconditional, so the fragment registers keep one
+                        // role around the loop; after the last stage it
+                        // re-reads the current slot and the values go unused
+                        read(pslot, 0, fa[0], fb[0]);
+                        __builtin_amdgcn_sched_barrier(0);
+                        mfma(fa[cur], fb[cur], QH{}, QE{});
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
                 }
-                cs.advance();
+                if constexpr (FUSED) {
+                    if (cs.last_in_panel()) {
+                        // the panel's partial joins the running sum: C += A_p . B_p
+#pragma unroll
+                        for (int i = 0; i < FI; ++i)
+#pragma unroll
+                            for (int j = 0; j < FJ; ++j) {
+                                csum[i][j] += acc[i][j];
+                                acc[i][j] = vzero;
+                            }
+                    }
+                    cs.advance();
+                }
             }
         }
 
@@ -351,9 +445,15 @@ __global__ __launch_bounds__(S::THREADS, 1) void dgemm_panels(PanelArgs a) {
                 for (int r = 0; r < 4; ++r) {
                     const int64_t row = m0 + wm * S::WTM + 16 * i + 4 * r + lg;
                     const int64_t col = n0 + wn * S::WTN + 16 * j + lc;
-                    if (row < M && col < N) a.C[row * a.ldc + col] = csum[i][j][r];
+                    if (row < M && col < N) {
+                        double *c = a.C + row * a.ldc + col;
+                        if constexpr (FUSED)
+                            *c = csum[i][j][r];
+                        else
+                            *c = a.accumulate ? *c + acc[i][j][r] : 0.0 + acc[i][j][r];
+                    }
                 }
-        // loads past the last stage and the C stores retire, and every wave
+        // loads past the last stage and the C traffic retire, and every wave
         // is done with the ring, before the next tile reuses it
         wait_vmcnt<0>();
         block_barrier();
@@ -376,11 +476,12 @@ int cu_count() {
     return cached[dev];
 }
 
-template <class S>
+template <class S, bool FUSED>
 int launch_panels(const double *A, const double *B, double *C, int64_t m, int64_t n, int64_t k,
                   int64_t panel, int64_t lda, int64_t ldb, int64_t ldc, bool accumulate,
                   hipStream_t st) {
     const int64_t tm = (m + S::BM - 1) / S::BM, tn = (n + S::BN - 1) / S::BN;
+    MGX_ARG_CHECK(k <= (1ll << 30), "k too large for one launch (at most 2^30)");
     MGX_ARG_CHECK(tm * tn <= 0x7fffffffll && tm <= 0x7fffffff && tn <= 0x7fffffff,
                   "matrix too large for one launch");
     PanelArgs a;
@@ -395,14 +496,16 @@ int launch_panels(const double *A, const double *B, double *C, int64_t m, int64_
     a.ldc = ldc;
     a.panel = panel < k ? panel : (k > 0 ? k : 1);
     // k stages: whole panels of ceil(panel / KS) stages, then the ragged one
-    const int64_t spp = (a.panel + KS - 1) / KS;
-    a.stages = k > 0 ? (k / a.panel) * spp + ((k % a.panel) + KS - 1) / KS : 0;
+    const int64_t spp = (a.panel + S::KS - 1) / S::KS;
+    a.stages = k > 0 ? (k / a.panel) * spp + ((k % a.panel) + S::KS - 1) / S::KS : 0;
     a.accumulate = accumulate ? 1 : 0;
     // direct loads need 16-byte aligned A and B rows and stage starts: even
     // leading dimensions, even N (the last B pair of a row stays in the row)
     // and even panel starts
     a.dma = (((uintptr_t)A | (uintptr_t)B) & 15) == 0 && (lda & 1) == 0 && (ldb & 1) == 0 &&
-            (n & 1) == 0 && ((a.panel & 1) == 0 || a.panel >= k);
+            (n & 1) == 0 && ((a.panel & 1) == 0 || a.panel >= k) &&
+            // per-lane byte offsets within a tile fit 32 bits
+            (int64_t)S::BM * lda * 8 < (1ll << 31) && (int64_t)S::KS * ldb * 8 < (1ll << 31);
     a.tiles_m = (int)tm;
     a.tiles_n = (int)tn;
     const int64_t tiles = tm * tn;
@@ -410,7 +513,7 @@ int launch_panels(const double *A, const double *B, double *C, int64_t m, int64_
     const int per_cu = (S::LDS_BYTES > 80 * 1024) ? 1 : 2;
     const int64_t slots = (int64_t)cu_count() * per_cu;
     const int64_t grid = tiles <= slots ? tiles : (slots / 8) * 8;
-    hipLaunchKernelGGL((dgemm_panels<S>), dim3((unsigned)grid), dim3(S::THREADS), 0, st, a);
+    hipLaunchKernelGGL((dgemm_panels<S, FUSED>), dim3((unsigned)grid), dim3(S::THREADS), 0, st, a);
     return mgx::launch_status("dgemm_panels");
 }
 
@@ -418,11 +521,20 @@ int launch_dgemm(int tile, bool accumulate, const double *A, const double *B, do
                  int64_t m, int64_t n, int64_t k, int64_t panel, int64_t lda, int64_t ldb,
                  int64_t ldc, hipStream_t st) {
     if (tile == MGX_TILE_DEFAULT) tile = MGX_TILE_LARGE;
+    const bool one_panel = panel >= k;
     switch (tile) {
         case MGX_TILE_LARGE:
-            return launch_panels<Large>(A, B, C, m, n, k, panel, lda, ldb, ldc, accumulate, st);
+            if (one_panel && MGX_DG_MONO == 2)
+                return launch_panels<Wide, false>(A, B, C, m, n, k, panel, lda, ldb, ldc,
+                                                  accumulate, st);
+            if (one_panel && MGX_DG_MONO == 1)
+                return launch_panels<Large, false>(A, B, C, m, n, k, panel, lda, ldb, ldc,
+                                                   accumulate, st);
+            return launch_panels<Large, true>(A, B, C, m, n, k, panel, lda, ldb, ldc, accumulate,
+                                              st);
         case MGX_TILE_SMALL:
-            return launch_panels<Small>(A, B, C, m, n, k, panel, lda, ldb, ldc, accumulate, st);
+            return launch_panels<Small, true>(A, B, C, m, n, k, panel, lda, ldb, ldc, accumulate,
+                                              st);
         default:
             return mgx::set_error(MGX_ERR_UNSUPPORTED, "unknown tile");
     }
