@@ -24,7 +24,7 @@ LIB = PKG / "libmgx.so"
 INCLUDE = PKG.parent / "include"
 
 LIB_SOURCES = ["runtime.hip", "dgemm.hip", "ozaki.hip", "summa.hip", "accuracy.hip", "mlp.hip", "tune.hip",
-               "ceiling.hip"]
+               "ceiling.hip", "noalloc.hip"]
 EXE_SOURCES = {"mm_1d": "mm_1d.cpp", "mm_2d": "mm_2d.cpp"}  # linked against libmgx.so
 
 ARCH = "gfx950"
