@@ -92,7 +92,8 @@ def _library_versions() -> dict:
         "rccl_header_version": _header_version(rccl, "NCCL_"),
         "rccl_header_path": str(rccl) if rccl.is_file() else None,
         "hip_header_version": hip_ver,
-        "note": "Header versions are compile-time hints, not proof of the loaded runtime versions.",
+        "note": ("read from the installed headers: the versions this build compiles against, "
+                 "which need not be the libraries loaded at run time"),
     }
 
 
@@ -114,12 +115,13 @@ def collect_environment(metadata_path: str | None = None, build_dir: str | None 
                     and not any(x in k.upper() for x in ("TOKEN", "KEY", "PASSWORD", "SECRET"))},
         "allocation": {k: os.environ[k] for k in ("SLURM_JOB_NODELIST", "SLURM_NNODES",
                                                    "SLURM_GPUS_ON_NODE") if k in os.environ},
-        "software_note": "RCCL/HIP header versions are hints; capture runtime versions separately.",
+        "software_note": ("library_versions come from headers; record the loaded runtime "
+                          "versions separately when they matter"),
     }
     if metadata_path:
         supplied = json.loads(Path(metadata_path).read_text())
         if not isinstance(supplied, dict):
-            raise ValueError("environment metadata must be a JSON object")
+            raise ValueError(f"{metadata_path}: expected a JSON object of environment fields")
         data["user"] = supplied
     if build_dir:
         data["binaries"] = {}
