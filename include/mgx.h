@@ -236,16 +236,18 @@ int mgx_error_report(const double *A, const double *B, const double *C,
  * MPI/NCCL/NVSHMEM broadcasts of the reference are replaced by these
  * host-to-device copies (the single-GPU stand-in).
  * Host strips: pinned (mgx_host_alloc) strips are gathered directly by
- * strided DMA; pageable strips are packed into library-owned pinned
- * buffers first (the reference's pack_a / pack_b).  dC is a device m x n
- * matrix that is accumulated into (zero it for C = A.B).  Synchronous: all
- * resources are created before and released after the timed loop.
- * Consecutive panels are gathered into a group of mgx_split_group_k(...)
- * columns (whole panels; with pinned strips and asynchronous copies one A and
- * one B copy per group) and the group is multiplied by one launch:
- * FP64 by mgx_dgemm_panels (one partial per panel, the reference's result),
- * split forms by one split product C += A_g . B_g, so the exponent scan,
- * operand conversion and reconstruction are paid per group, not per panel. */
+ * strided DMA, one A copy and one B copy per group; pageable strips are
+ * packed (the reference's pack_a / pack_b) by a pool of host_threads host
+ * threads into two library-owned pinned staging buffers of whole panels
+ * (up to 1024 columns), packing the next unit while the current one is
+ * copied.  dC is a device m x n matrix that is accumulated into (zero it for
+ * C = A.B).  Synchronous: all resources (buffers, streams, host threads) are
+ * created before and released after the timed loop.
+ * Consecutive panels form groups (mgx_summa_groups) and each group is one
+ * launch: FP64 by mgx_dgemm_panels (one partial per panel, the reference's
+ * result), split forms by one split product C += A_g . B_g, so the exponent
+ * scan, operand conversion and reconstruction are paid per group, not per
+ * panel.  The copy of a group overlaps the product of the previous one. */
 enum {
     MGX_SUMMA_COPY_ASYNC = 0,      /* copies overlap compute (MMM, YYY, NNN codes) */
     MGX_SUMMA_COPY_HOST_SYNC = 1   /* host waits for each panel copy (CCC, XXX, WWW, SSS) */
@@ -259,28 +261,49 @@ typedef struct mgx_summa_params {
     int copy_mode;      /* MGX_SUMMA_COPY_* */
     int use_graph;      /* capture the whole panel loop in one hipGraph (pinned strips) */
     int serialize;      /* debug: synchronise after every copy and every product */
-    int64_t split_k;    /* columns per launch (whole panels); <= 0 selects the
-                           form's default group (mgx_split_group_k) */
+    int64_t split_k;    /* group schedule (mgx_summa_groups): > 0 uniform groups of
+                           this many columns (whole panels), 0 the default width,
+                           < 0 the tapered schedule on groups of -split_k */
+    int host_threads;   /* threads packing pageable strips, 1..64; 0 = automatic
+                           (min(16, hardware threads)); unused with pinned strips */
+    int tile;           /* MGX_TILE_* block tile of the group products */
 } mgx_summa_params;
 
 typedef struct mgx_summa_stats {
     double seconds;     /* host wall time of the panel loop (launch to completion) */
     int64_t panels;     /* number of panels, including a ragged last one */
     int host_pinned;    /* 1 if the host strips were pinned (no host packing) */
+    int host_threads;   /* host threads that packed pageable strips (1 when pinned) */
+    int64_t groups;     /* launches (panel groups) of the loop */
 } mgx_summa_stats;
 
-/* Default reduction columns per launch in a panel loop: split products, and
- * FP64 panel groups (smaller: an FP64 group only saves the C traffic, and a
- * streamed loop ends with the last group's product). */
-#define MGX_SPLIT_GROUP_K 4096
-#define MGX_FP64_GROUP_K 2048
+/* Default reduction columns per launch in a panel loop, split products and
+ * FP64 panel groups.  A streamed loop ends with its last group's product, and
+ * narrow groups keep that short; their fixed costs (split forms: exponent
+ * scan and reconstruction) hide under the next group's copy.  Measured on
+ * the 8192 x 8192 x 16384 share with pinned strips: 1024 columns take
+ * 40.1 ms (15 moduli) and 40.2 ms (FP64), against 43.1 and 42.0 ms with the
+ * 4096 / 2048 groups. */
+#define MGX_SPLIT_GROUP_K 1024
+#define MGX_FP64_GROUP_K 1024
 
-/* Reduction columns of each launch in a panel loop (SUMMA and the
- * autotuner's local product): the smallest whole number of panels covering
- * split_k columns (MGX_SPLIT_GROUP_K, or MGX_FP64_GROUP_K for MGX_FORM_FP64,
- * when split_k <= 0), capped at k.  The last group may be shorter.  Returns
- * -1 for non-positive panel or k, or an unknown form. */
+/* Group width of uniform panel groups: the smallest whole number of panels
+ * covering split_k columns (MGX_SPLIT_GROUP_K, or MGX_FP64_GROUP_K for
+ * MGX_FORM_FP64, when split_k <= 0), capped at k.  Returns -1 for
+ * non-positive panel or k, or an unknown form. */
 int64_t mgx_split_group_k(int form, int64_t panel, int64_t split_k, int64_t k);
+
+/* The group schedule of mgx_summa_run: column widths of its launches, in
+ * order (sum k).  split_k >= 0: uniform groups of
+ * mgx_split_group_k(form, panel, split_k, k) columns, the last one shorter.
+ * split_k < 0, the tapered schedule: groups of G = mgx_split_group_k(form,
+ * panel, -split_k, k) columns (P panels), except that when k > G the loop
+ * ends with T = min(4, P / t) groups of t = max(1, P / 4) panels, so the
+ * product left after the last copy is short (worth it for wide groups).
+ * The ragged last panel ends the last group.  Writes min(count, capacity) widths (widths may be NULL) and
+ * returns the group count, or -1 as mgx_split_group_k. */
+int64_t mgx_summa_groups(int form, int64_t panel, int64_t split_k, int64_t k,
+                         int64_t *widths, int64_t capacity);
 
 int mgx_summa_run(const mgx_summa_params *params,
                   const double *hA, int64_t lda,
@@ -339,7 +362,7 @@ double mgx_mfma_i8_ceiling_ops(int blocks, int iters);
  * power law, lstsq "physical" fit, ridge "ml" fit).  Parameters are one FP32
  * vector: W1 [in_dim][64], b1 [64], W2 [64][64], b2 [64], W3 [64], b3.
  * All arrays are host arrays; calls are synchronous. */
-#define MLP_MAX_IN 16
+#define MLP_MAX_IN 24
 #define MLP_HIDDEN 64
 int64_t mgx_mlp_param_count(int in_dim);                 /* -1 if in_dim is out of range */
 int mgx_mlp_init(float *theta, int in_dim, uint64_t seed);  /* He-uniform weights, zero biases */

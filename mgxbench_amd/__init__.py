@@ -21,7 +21,7 @@ from ._lib import FORM_BF16, FORM_FP64, FORM_INT8, FORM_INT8_MOD, MgxError, lib
 
 __all__ = [
     "ABMultiply", "ABMultiplyAccumulateAsync", "validate_matrix_C", "split_gemm",
-    "split_gemm_events", "split_gemm_noalloc", "split_workspace_bytes", "error_report", "summa_run", "split_group_k", "fill", "fill_exponents", "scale_rows_cols", "version", "peak",
+    "split_gemm_events", "split_gemm_noalloc", "split_workspace_bytes", "error_report", "summa_run", "split_group_k", "summa_groups", "fill", "fill_exponents", "scale_rows_cols", "version", "peak",
     "FORM_FP64", "FORM_INT8", "FORM_INT8_MOD", "FORM_BF16", "MgxError", "lib",
 ]
 
@@ -120,27 +120,43 @@ def error_report(a, b, c, m, n, k, lda, ldb, ldc, rows) -> dict:
 
 
 def summa_run(h_a, lda, h_b, ldb, d_c, ldc, *, m, n, k, panel, form=FORM_FP64, slices=8,
-              copy_mode=0, use_graph=False, serialize=False, split_k=0) -> dict:
+              copy_mode=0, use_graph=False, serialize=False, split_k=0,
+              host_threads=0, tile=0) -> dict:
     """One rank's SUMMA panel loop: C += sum_p A_p . B_p with panels streamed
     from host strips (pointers as ints or CPU tensors; pinned ones are
-    gathered by DMA directly).  Split forms multiply groups of whole panels
-    covering split_k columns (0: the library default) per split product.
-    Returns the loop time and panel count."""
+    gathered by DMA directly, pageable ones packed by `host_threads` host
+    threads, 0 = automatic).  Panels are multiplied in groups (summa_groups;
+    split_k > 0 asks for uniform groups of that many columns) by products on
+    block tile `tile` (TILE_*).  Returns the loop time, panel and group counts."""
     import ctypes
     P = _lib.SummaParams(m, n, k, panel, form, slices, copy_mode, 1 if use_graph else 0,
-                         1 if serialize else 0, split_k)
+                         1 if serialize else 0, split_k, host_threads, tile)
     S = _lib.SummaStats()
     _lib.call("mgx_summa_run", ctypes.byref(P), _ptr(h_a), lda, _ptr(h_b), ldb, _ptr(d_c), ldc,
               ctypes.byref(S))
-    return {"seconds": S.seconds, "panels": S.panels, "host_pinned": S.host_pinned}
+    return {"seconds": S.seconds, "panels": S.panels, "host_pinned": S.host_pinned,
+            "host_threads": S.host_threads, "groups": S.groups}
 
 
 def split_group_k(form, panel, k, split_k=0) -> int:
-    """Reduction columns per product in a panel loop (mgx_split_group_k)."""
+    """Columns of a uniform panel group (mgx_split_group_k)."""
     g = int(_lib.lib().mgx_split_group_k(form, panel, split_k, k))
     if g < 0:
-        raise MgxError(f"invalid panel loop: form={form} panel={panel} k={k}")
+        raise MgxError(-1, "mgx_split_group_k", f"invalid panel loop: form={form} "
+                       f"panel={panel} k={k}")
     return g
+
+
+def summa_groups(form, panel, k, split_k=0) -> list[int]:
+    """Column widths of the panel loop's launches, in order (mgx_summa_groups)."""
+    import ctypes
+    count = int(_lib.lib().mgx_summa_groups(form, panel, split_k, k, None, 0))
+    if count < 0:
+        raise MgxError(-1, "mgx_summa_groups", f"invalid panel loop: form={form} "
+                       f"panel={panel} k={k}")
+    out = (ctypes.c_int64 * count)()
+    _lib.lib().mgx_summa_groups(form, panel, split_k, k, out, count)
+    return list(out)
 
 
 def fill(x, rows, cols, ld, kind, seed=0, value=0.0, stream=None) -> None:

@@ -42,12 +42,14 @@ class SummaParams(ctypes.Structure):
     _fields_ = [("m", ctypes.c_int64), ("n", ctypes.c_int64), ("k", ctypes.c_int64),
                 ("panel", ctypes.c_int64), ("form", ctypes.c_int), ("slices", ctypes.c_int),
                 ("copy_mode", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("serialize", ctypes.c_int), ("split_k", ctypes.c_int64)]
+                ("serialize", ctypes.c_int), ("split_k", ctypes.c_int64),
+                ("host_threads", ctypes.c_int), ("tile", ctypes.c_int)]
 
 
 class SummaStats(ctypes.Structure):
     _fields_ = [("seconds", ctypes.c_double), ("panels", ctypes.c_int64),
-                ("host_pinned", ctypes.c_int)]
+                ("host_pinned", ctypes.c_int), ("host_threads", ctypes.c_int),
+                ("groups", ctypes.c_int64)]
 
 
 class TuneConfig(ctypes.Structure):
@@ -90,6 +92,7 @@ _SIGNATURES = {
     "mgx_error_report": (_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _i64,
                                 ctypes.POINTER(ErrorStats)]),
     "mgx_split_group_k": (_i64, [_int, _i64, _i64, _i64]),
+    "mgx_summa_groups": (_i64, [_int, _i64, _i64, _i64, _vp, _i64]),
     "mgx_summa_run": (_int, [ctypes.POINTER(SummaParams), _vp, _i64, _vp, _i64, _vp, _i64,
                              ctypes.POINTER(SummaStats)]),
     "mgx_mfma_i8_ceiling": (_int, [_vp, _int, _int, _int, _vp]),

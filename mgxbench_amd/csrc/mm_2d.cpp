@@ -10,7 +10,8 @@
 // overlapped with the accumulate product on a compute stream (the single-GPU
 // stand-in for the per-panel broadcasts).  As in the reference's timed loop
 // (2D-block-matrix/mmb.cu:264-265), the strips are pageable and every panel
-// is packed on the host into a pinned staging buffer before its copy.  With
+// is packed on the host into a pinned staging buffer before its copy (here
+// by a pool of host threads, MGX_HOST_THREADS, default min(16, cores)).  With
 // one rank, every library code moves panels host -> device; MMM/YYY/NNN
 // overlap the copies, CCC/XXX/WWW/SSS wait for each copy on the host, as the
 // reference's P=1 paths do.
@@ -122,7 +123,7 @@ int main(int argc, char **argv) {
     if (hipMalloc((void **)&dC, sizeof(double) * m * n) != hipSuccess) fail("hipMalloc failed");
     if (mgx_fill(dC, m, n, n, 0, 0, 0.0, nullptr) != MGX_OK) fail(mgx_last_error());
 
-    mgx_summa_params P;
+    mgx_summa_params P{};
     P.m = m;
     P.n = n;
     P.k = N;
@@ -135,6 +136,7 @@ int main(int argc, char **argv) {
     P.use_graph = graph ? 1 : 0;
     P.serialize = env_int("MGX_SUMMA_SERIALIZE", 0) != 0;
     P.split_k = env_int("MGX_SPLIT_K", 0);
+    P.host_threads = env_int("MGX_HOST_THREADS", 0);
     mgx_summa_stats S;
     if (mgx_summa_run(&P, hA, N, hB, n, dC, n, &S) != MGX_OK) fail(mgx_last_error());
     const double elapsed = S.seconds;

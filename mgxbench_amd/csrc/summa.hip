@@ -6,38 +6,174 @@
 // p of A is an m x kc column block of the host A strip, panel p of B is kc
 // rows of the host B strip.  They reach the device by hipMemcpy2DAsync /
 // hipMemcpyAsync on a transfer stream; the compute stream waits on a
-// per-buffer "panel ready" event and runs the accumulate product; a
-// "compute done" event guards reuse of each of the two device buffers.
+// per-buffer "group ready" event and runs the product; a "compute done"
+// event guards reuse of each of the two device group buffers.
 //
-// Consecutive panels are gathered into a group buffer (whole panels covering
-// mgx_split_group_k columns) and each group is one launch: FP64 runs the
-// panel-fused kernel (mgx_dgemm_panels: one partial per panel added into the
-// register-held C tile, so C is read and written once per group); split
-// forms run one split product per group, so the exponent scan, operand
-// conversion and (modular form) reconstruction of the whole C are paid once
-// per group.  The copies overlap the previous group's product.  With pinned
-// strips and asynchronous copies a group arrives as one A copy and one B
-// copy; otherwise panel by panel, the reference's granularity.
+// Consecutive panels form groups (whole panels, mgx_summa_groups) and each
+// group is one launch: FP64 runs the panel-fused kernel (mgx_dgemm_panels:
+// one partial per panel added into the register-held C tile, so C is read
+// and written once per group); split forms run one split product per group,
+// so the exponent scan, operand conversion and (modular form)
+// reconstruction of the whole C are paid once per group.  A group's copies
+// overlap the previous group's product, so a streamed loop ends with the last
+// group's product: the default groups are narrow (MGX_SPLIT_GROUP_K,
+// MGX_FP64_GROUP_K), and a tapered schedule (narrower last groups) is
+// available for wide ones.
 //
-// MI355X-native differences from the reference's loop:
-//  * With pinned host strips the A panel is gathered by the DMA engine
-//    (strided 2D copy) instead of host memcpy packing; with pageable strips
-//    the library packs into its own pinned double buffer as the reference
-//    does (pack_a / pack_b, 2D-block-matrix/mmb.cu:60-72).
-//  * Buffer reuse is ordered by stream waits on events, not host syncs, so
-//    the whole loop can be captured into one hipGraph and launched once.
+// Host strips:
+//  * pinned: each group arrives as one A copy (a strided 2D copy of the
+//    group's columns) and one B copy, gathered by the DMA engine;
+//  * pageable (the reference's case, pack_a / pack_b,
+//    2D-block-matrix/mmb.cu:60-72, 264-265): whole panels of up to
+//    PACK_UNIT_K columns are packed into one of two pinned staging buffers by
+//    a pool of host threads, then copied; packing the next unit overlaps the
+//    copy of the current one.
+// With host-synchronous copies (the CCC/XXX/WWW/SSS codes) or `serialize`,
+// panels move one at a time and the host waits for each copy, as the
+// reference's P=1 paths do.
+//
+// Buffer reuse is ordered by stream waits on events, not host syncs, so with
+// pinned strips the whole loop can be captured into one hipGraph and
+// launched once.
 #include "mgx_internal.hpp"
 
+#include <emmintrin.h>
+
+#include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <vector>
 
-extern "C" int mgx_split_gemm(int form, int slices, const double *A, const double *B, double *C,
-                              int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb,
-                              int64_t ldc, int accumulate, void *workspace,
-                              size_t workspace_bytes, void *stream);
-
 namespace {
+
+// Columns packed per pinned staging unit with pageable strips (whole panels;
+// a single wider panel is its own unit).
+constexpr int64_t PACK_UNIT_K = 1024;
+constexpr int MAX_HOST_THREADS = 64;
+
+bool form_known(int form) {
+    return form == MGX_FORM_FP64 || form == MGX_FORM_INT8 || form == MGX_FORM_INT8_MOD ||
+           form == MGX_FORM_BF16;
+}
+
+// Group widths of a panel loop over k columns (see mgx_summa_groups).
+std::vector<int64_t> summa_schedule(int form, int64_t panel, int64_t split_k, int64_t k) {
+    std::vector<int64_t> widths;
+    const int64_t p = panel < k ? panel : k;
+    const int64_t total = (k + p - 1) / p;                                 // panels
+    const int64_t width = mgx_split_group_k(form, p, split_k < 0 ? -split_k : split_k, k);
+    const int64_t per = (width + p - 1) / p;                               // panels per group
+    const bool taper = split_k < 0;
+    int64_t tail_groups = 0, tail_panels = 0;
+    if (taper && total > per) {
+        tail_panels = per / 4 > 0 ? per / 4 : 1;
+        tail_groups = std::min<int64_t>(4, per / tail_panels);
+    }
+    const int64_t body = total - tail_groups * tail_panels;
+    for (int64_t done = 0; done < body; done += per) widths.push_back(std::min(per, body - done));
+    for (int64_t t = 0; t < tail_groups; ++t) widths.push_back(tail_panels);
+    // panels -> columns; the last panel may be ragged
+    for (auto &w : widths) w *= p;
+    widths.back() -= total * p - k;
+    return widths;
+}
+
+// A small pool of host threads for packing pageable strips.  run(job) calls
+// job(i, count) once on every thread (the caller is thread 0) and returns
+// when all calls have returned.  If the system refuses a thread, the pool
+// works with the threads it has.
+class HostPool {
+   public:
+    explicit HostPool(int threads) {
+        try {
+            for (int i = 1; i < threads; ++i) {
+                workers_.emplace_back([this, i] { serve(i); });
+                ++count_;
+            }
+        } catch (...) {
+        }
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            quit_ = true;
+        }
+        go_.notify_all();
+        for (auto &t : workers_) t.join();
+    }
+    int count() const { return count_; }
+    void run(const std::function<void(int, int)> &job) {
+        if (count_ == 1) {
+            job(0, 1);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &job;
+            pending_ = count_ - 1;
+            ++generation_;
+        }
+        go_.notify_all();
+        job(0, count_);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [this] { return pending_ == 0; });
+        job_ = nullptr;
+    }
+
+   private:
+    void serve(int index) {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            go_.wait(lk, [&] { return quit_ || generation_ != seen; });
+            if (quit_) return;
+            seen = generation_;
+            const std::function<void(int, int)> *job = job_;
+            lk.unlock();
+            (*job)(index, count_);
+            lk.lock();
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    int count_ = 1;
+    std::vector<std::thread> workers_;
+    std::mutex mu_;
+    std::condition_variable go_, done_;
+    const std::function<void(int, int)> *job_ = nullptr;
+    uint64_t generation_ = 0;
+    int pending_ = 0;
+    bool quit_ = false;
+};
+
+// Copies `count` doubles with streaming (non-temporal) stores: the pinned
+// staging buffer is read next by the DMA engine, not by this core, so the
+// stores skip the cache and the read-for-ownership of every line.  The
+// caller issues _mm_sfence() before publishing the data.
+void stream_copy(double *dst, const double *src, int64_t count) {
+    int64_t i = 0;
+    if (((uintptr_t)dst & 15) != 0 && count > 0) {
+        dst[0] = src[0];
+        i = 1;
+    }
+    for (; i + 8 <= count; i += 8) {
+        const __m128d a = _mm_loadu_pd(src + i), b = _mm_loadu_pd(src + i + 2);
+        const __m128d c = _mm_loadu_pd(src + i + 4), d = _mm_loadu_pd(src + i + 6);
+        _mm_stream_pd(dst + i, a);
+        _mm_stream_pd(dst + i + 2, b);
+        _mm_stream_pd(dst + i + 4, c);
+        _mm_stream_pd(dst + i + 6, d);
+    }
+    for (; i < count; ++i) dst[i] = src[i];
+}
+
+int default_host_threads() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    return hw == 0 ? 4 : (int)std::min<unsigned>(hw, 16);
+}
 
 struct SummaResources {
     hipStream_t compute = nullptr, transfer = nullptr;
@@ -85,15 +221,21 @@ bool host_is_pinned(const void *p) {
 }  // namespace
 
 extern "C" int64_t mgx_split_group_k(int form, int64_t panel, int64_t split_k, int64_t k) {
-    if (panel <= 0 || k <= 0) return -1;
-    if (form != MGX_FORM_FP64 && form != MGX_FORM_INT8 && form != MGX_FORM_INT8_MOD &&
-        form != MGX_FORM_BF16)
-        return -1;
+    if (panel <= 0 || k <= 0 || !form_known(form)) return -1;
     const int64_t p = panel < k ? panel : k;
     const int64_t want =
         split_k > 0 ? split_k : (form == MGX_FORM_FP64 ? MGX_FP64_GROUP_K : MGX_SPLIT_GROUP_K);
     const int64_t panels = (want + p - 1) / p;
     return panels * p < k ? panels * p : k;
+}
+
+extern "C" int64_t mgx_summa_groups(int form, int64_t panel, int64_t split_k, int64_t k,
+                                    int64_t *widths, int64_t capacity) {
+    if (panel <= 0 || k <= 0 || !form_known(form)) return -1;
+    const std::vector<int64_t> w = summa_schedule(form, panel, split_k, k);
+    if (widths)
+        for (int64_t i = 0; i < (int64_t)w.size() && i < capacity; ++i) widths[i] = w[i];
+    return (int64_t)w.size();
 }
 
 extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_t lda,
@@ -105,20 +247,26 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
     MGX_ARG_CHECK(m > 0 && n > 0 && k > 0 && bk > 0, "sizes and panel width must be positive");
     MGX_ARG_CHECK(hA && hB && dC, "NULL matrix pointer");
     MGX_ARG_CHECK(lda >= k && ldb >= n && ldc >= n, "leading dimension too small");
-    MGX_ARG_CHECK(P->form == MGX_FORM_FP64 || P->form == MGX_FORM_INT8 ||
-                      P->form == MGX_FORM_INT8_MOD || P->form == MGX_FORM_BF16,
-                  "unsupported form");
+    MGX_ARG_CHECK(form_known(P->form), "unsupported form");
     MGX_ARG_CHECK(P->copy_mode == MGX_SUMMA_COPY_ASYNC || P->copy_mode == MGX_SUMMA_COPY_HOST_SYNC,
                   "unknown copy mode");
+    MGX_ARG_CHECK(P->host_threads >= 0 && P->host_threads <= MAX_HOST_THREADS,
+                  "host_threads out of range");
+    MGX_ARG_CHECK(P->tile == MGX_TILE_DEFAULT || P->tile == MGX_TILE_LARGE ||
+                      P->tile == MGX_TILE_SMALL,
+                  "unknown tile");
     const int64_t panel = bk < k ? bk : k;
     const int64_t npanels = (k + panel - 1) / panel;
-    // group width: one panel (FP64) or whole panels per split product
-    const int64_t gw = mgx_split_group_k(P->form, panel, P->split_k, k);
-    const int64_t ngroups = (k + gw - 1) / gw;
+    const std::vector<int64_t> groups = summa_schedule(P->form, panel, P->split_k, k);
+    const int64_t gmax = *std::max_element(groups.begin(), groups.end());
     const bool pinned = host_is_pinned(hA) && host_is_pinned(hB);
     const bool graph = P->use_graph != 0;
-    MGX_ARG_CHECK(!graph || (pinned && P->copy_mode == MGX_SUMMA_COPY_ASYNC && !P->serialize),
+    const bool panelwise = P->copy_mode == MGX_SUMMA_COPY_HOST_SYNC || P->serialize;
+    MGX_ARG_CHECK(!graph || (pinned && !panelwise),
                   "hipGraph capture needs pinned host strips, async copies and no serialization");
+    // pageable strips: staging units of whole panels (one panel when the host
+    // waits for every copy)
+    const int64_t unit = panelwise ? panel : std::max(panel, PACK_UNIT_K / panel * panel);
 
     SummaResources R;
     MGX_HIP_TRY(hipStreamCreateWithFlags(&R.compute, hipStreamNonBlocking));
@@ -127,71 +275,79 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
         MGX_HIP_TRY(hipEventCreateWithFlags(&R.ready[i], hipEventDisableTiming));
         MGX_HIP_TRY(hipEventCreateWithFlags(&R.done[i], hipEventDisableTiming));
         MGX_HIP_TRY(hipEventCreateWithFlags(&R.staged[i], hipEventDisableTiming));
-        if (hipMalloc(&R.dA[i], sizeof(double) * m * gw) != hipSuccess ||
-            hipMalloc(&R.dB[i], sizeof(double) * gw * n) != hipSuccess)
+        if (hipMalloc(&R.dA[i], sizeof(double) * m * gmax) != hipSuccess ||
+            hipMalloc(&R.dB[i], sizeof(double) * gmax * n) != hipSuccess)
             return mgx::set_error(MGX_ERR_ALLOC, "panel buffer allocation failed");
         if (!pinned) {
-            if (hipHostMalloc((void **)&R.hA[i], sizeof(double) * m * panel) != hipSuccess ||
-                hipHostMalloc((void **)&R.hB[i], sizeof(double) * panel * n) != hipSuccess)
+            if (hipHostMalloc((void **)&R.hA[i], sizeof(double) * m * unit) != hipSuccess ||
+                hipHostMalloc((void **)&R.hB[i], sizeof(double) * unit * n) != hipSuccess)
                 return mgx::set_error(MGX_ERR_ALLOC, "pinned staging allocation failed");
         }
     }
     MGX_HIP_TRY(hipEventCreateWithFlags(&R.fork, hipEventDisableTiming));
     MGX_HIP_TRY(hipEventCreateWithFlags(&R.join, hipEventDisableTiming));
     if (P->form != MGX_FORM_FP64) {
-        R.ws_bytes = mgx_split_workspace_bytes(P->form, P->slices, m, n, gw);
+        R.ws_bytes = mgx_split_workspace_bytes(P->form, P->slices, m, n, gmax);
         if (R.ws_bytes == 0) return mgx::set_error(MGX_ERR_UNSUPPORTED, "slice count out of range");
         if (hipMalloc(&R.ws, R.ws_bytes) != hipSuccess)
             return mgx::set_error(MGX_ERR_ALLOC, "split workspace allocation failed");
     }
+    // host threads exist only for packing pageable strips
+    HostPool pool(pinned ? 1 : (P->host_threads > 0 ? P->host_threads : default_host_threads()));
 
-    // Enqueues the panel loop on the two streams.  Device group buffers are
-    // reused under stream waits on "compute done"; with pageable strips the
-    // host packs each panel into one of two pinned staging buffers after the
-    // copy that last read it has finished.
+    // Packs columns k0 .. k0+kc of the A strip and rows k0 .. k0+kc of the
+    // B strip into staging buffer s, rows dealt out to the pool's threads.
+    auto pack = [&](int s, int64_t k0, int64_t kc) {
+        double *sa = R.hA[s], *sb = R.hB[s];
+        pool.run([&](int t, int nt) {
+            for (int64_t i = m * t / nt; i < m * (t + 1) / nt; ++i)
+                stream_copy(sa + i * kc, hA + i * lda + k0, kc);
+            for (int64_t r = kc * t / nt; r < kc * (t + 1) / nt; ++r)
+                stream_copy(sb + r * n, hB + (k0 + r) * ldb, n);
+            _mm_sfence();
+        });
+    };
+
+    // Enqueues the panel loop on the two streams.
     auto enqueue = [&]() -> int {
         int rc;
         MGX_HIP_TRY(hipEventRecord(R.fork, R.compute));
         MGX_HIP_TRY(hipStreamWaitEvent(R.transfer, R.fork, 0));
-        int64_t pi = 0;   // panel counter (staging buffer parity)
-        for (int64_t g = 0; g < ngroups; ++g) {
+        int64_t staged_units = 0;   // staging buffer parity with pageable strips
+        int64_t g0 = 0;
+        for (size_t g = 0; g < groups.size(); ++g) {
             const int cur = (int)(g & 1);
-            const int64_t g0 = g * gw;
-            const int64_t gk = (k - g0) < gw ? (k - g0) : gw;
+            const int64_t gk = groups[g];
             if (g >= 2) MGX_HIP_TRY(hipStreamWaitEvent(R.transfer, R.done[cur], 0));
-            if (pinned && gk > panel && P->copy_mode == MGX_SUMMA_COPY_ASYNC && !P->serialize) {
-                // a split group's panels are adjacent columns of the A strip
-                // and adjacent rows of the B strip: one copy each (a copy
-                // costs ~50 us of idle copy engine whatever its size)
+            if (pinned && !panelwise) {
+                // the group's panels are adjacent columns of the A strip and
+                // adjacent rows of the B strip: one copy each
                 MGX_HIP_TRY(hipMemcpy2DAsync(R.dA[cur], sizeof(double) * gk, hA + g0,
                                              sizeof(double) * lda, sizeof(double) * gk, m,
                                              hipMemcpyHostToDevice, R.transfer));
                 MGX_HIP_TRY(hipMemcpy2DAsync(R.dB[cur], sizeof(double) * n, hB + g0 * ldb,
                                              sizeof(double) * ldb, sizeof(double) * n, gk,
                                              hipMemcpyHostToDevice, R.transfer));
-                pi += (gk + panel - 1) / panel;
             } else {
-                for (int64_t j0 = 0; j0 < gk; j0 += panel, ++pi) {
-                    const int64_t kc = (gk - j0) < panel ? (gk - j0) : panel;
+                const int64_t step = pinned ? panel : unit;
+                for (int64_t j0 = 0; j0 < gk; j0 += step) {
+                    const int64_t kc = (gk - j0) < step ? (gk - j0) : step;
                     const int64_t k0 = g0 + j0;
                     const double *srcA = hA + k0;
                     const double *srcB = hB + k0 * ldb;
                     int64_t spitchA = lda, spitchB = ldb;
-                    const int s = (int)(pi & 1);
+                    const int s = (int)(staged_units & 1);
                     if (!pinned) {
-                        // pack_a / pack_b into the pinned staging buffer
-                        if (pi >= 2) MGX_HIP_TRY(hipEventSynchronize(R.staged[s]));
-                        for (int64_t i = 0; i < m; ++i)
-                            std::memcpy(R.hA[s] + i * kc, hA + i * lda + k0, sizeof(double) * kc);
-                        for (int64_t r = 0; r < kc; ++r)
-                            std::memcpy(R.hB[s] + r * n, hB + (k0 + r) * ldb, sizeof(double) * n);
+                        // the copy that last read this staging buffer is done
+                        if (staged_units >= 2) MGX_HIP_TRY(hipEventSynchronize(R.staged[s]));
+                        pack(s, k0, kc);
                         srcA = R.hA[s];
                         srcB = R.hB[s];
                         spitchA = kc;
                         spitchB = n;
+                        ++staged_units;
                     }
-                    // panel j0 of the group: columns j0.. of the m x gk A buffer
-                    // (lda = gk) and rows j0.. of the gk x n B buffer
+                    // columns j0.. of the m x gk A buffer, rows j0.. of the gk x n B buffer
                     MGX_HIP_TRY(hipMemcpy2DAsync(R.dA[cur] + j0, sizeof(double) * gk, srcA,
                                                  sizeof(double) * spitchA, sizeof(double) * kc, m,
                                                  hipMemcpyHostToDevice, R.transfer));
@@ -199,23 +355,23 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
                                                  sizeof(double) * spitchB, sizeof(double) * n, kc,
                                                  hipMemcpyHostToDevice, R.transfer));
                     if (!pinned) MGX_HIP_TRY(hipEventRecord(R.staged[s], R.transfer));
-                    if (P->copy_mode == MGX_SUMMA_COPY_HOST_SYNC || P->serialize)
-                        MGX_HIP_TRY(hipStreamSynchronize(R.transfer));
+                    if (panelwise) MGX_HIP_TRY(hipStreamSynchronize(R.transfer));
                 }
             }
             MGX_HIP_TRY(hipEventRecord(R.ready[cur], R.transfer));
             MGX_HIP_TRY(hipStreamWaitEvent(R.compute, R.ready[cur], 0));
             if (P->form == MGX_FORM_FP64) {
                 // one launch per group, one partial per panel inside it
-                rc = mgx_dgemm_panels(MGX_TILE_DEFAULT, 1, R.dA[cur], R.dB[cur], dC, m, n, gk,
-                                      panel, gk, n, ldc, R.compute);
+                rc = mgx_dgemm_panels(P->tile, 1, R.dA[cur], R.dB[cur], dC, m, n, gk, panel, gk,
+                                      n, ldc, R.compute);
             } else {
-                rc = mgx_split_gemm(P->form, P->slices, R.dA[cur], R.dB[cur], dC, m, n, gk,
-                                    gk, n, ldc, 1, R.ws, R.ws_bytes, R.compute);
+                rc = mgx_split_gemm_ex(P->form, P->slices, P->tile, R.dA[cur], R.dB[cur], dC, m,
+                                       n, gk, gk, n, ldc, 1, R.ws, R.ws_bytes, R.compute);
             }
             if (rc != MGX_OK) return rc;
             MGX_HIP_TRY(hipEventRecord(R.done[cur], R.compute));
             if (P->serialize) MGX_HIP_TRY(hipStreamSynchronize(R.compute));
+            g0 += gk;
         }
         MGX_HIP_TRY(hipEventRecord(R.join, R.transfer));
         MGX_HIP_TRY(hipStreamWaitEvent(R.compute, R.join, 0));
@@ -249,6 +405,8 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
         stats->seconds = std::chrono::duration<double>(t1 - t0).count();
         stats->panels = npanels;
         stats->host_pinned = pinned ? 1 : 0;
+        stats->groups = (int64_t)groups.size();
+        stats->host_threads = pool.count();
     }
     return MGX_OK;
 }

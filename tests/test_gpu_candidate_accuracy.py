@@ -3,13 +3,15 @@ accuracy-stress inputs, not only on uniform ones.
 
 Inputs: random sign x U[1,2) x 2^U(-30,30), rows of A and columns of B further
 x 2^U(-10,10) (the accuracy-stress configuration), m = n = 512 and
-K = 4096, 16384, 32768.  Each candidate runs the autotuner's local product
-(mgx_time_local_product: the SUMMA loop's compute side, split products over
-groups of whole panels), and its normwise and componentwise errors against
-the GPU double-double product over all rows must not exceed those of the
-plain FP64 MFMA product of the same inputs.  The monolithic modular product
-(one split product over the whole depth, as bench.py times it) is held to the
-same bar at K = 4096 and 16384 (beta 52 and 51).
+K = 4096, 16384, 32768.  Each distinct (form, count, launch depth) of the
+candidates of both targets runs the autotuner's local product
+(mgx_time_local_product: split products over the candidate's launches - the
+whole k for resident operands, groups of whole panels when streamed), and
+its normwise and componentwise errors against the GPU double-double product
+over all rows must not exceed those of the plain FP64 MFMA product of the
+same inputs.  The monolithic modular product (one split product over the
+whole depth, as bench.py times it) is held to the same bar at K = 4096 and
+16384 (beta 52 and 51).
 """
 import ctypes
 
@@ -49,14 +51,15 @@ def test_autotuner_split_candidates_no_worse_than_plain(mgx, k):
     plain = mgx.error_report(A, B, C, m, n, k, k, n, n, rows)
     assert plain["normwise"] < 1e-13
     seen = set()
-    for c in at.candidate_space(k):
+    for c in at.candidate_space(k, "resident") + at.candidate_space(k, "streamed"):
         if c.form == "fp64":
             continue
         depth = at.step_k(c, k)
         if (c.form, c.slices, depth) in seen:
             continue
         seen.add((c.form, c.slices, depth))
-        cfg = mgx._lib.TuneConfig(at.FORM_CODE[c.form], c.slices, c.tile, c.panel)
+        cfg = mgx._lib.TuneConfig(at.FORM_CODE[c.form], c.slices, c.tile, c.panel,
+                                  c.group or k)
         ms = ctypes.c_double()
         C.fill_(float("nan"))
         mgx._lib.call("mgx_time_local_product", ctypes.byref(cfg), A.data_ptr(), B.data_ptr(),

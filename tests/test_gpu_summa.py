@@ -122,9 +122,11 @@ def test_panel_width_1024(mgx, orc, form):
             bound += 2 * (kc * u / (1 - kc * u)) * absp + 4 * u * (np.abs(ref) + absp)
         assert np.all(np.abs(C - ref) <= bound)
     else:
-        g = mgx.split_group_k(code, panel, k)
-        assert g == 4096
-        ref = _grouped(A, B, g, lambda a, b, C: orc.crt_gemm(a, b, 15, C=C))
+        # default groups of 1024 columns: every panel is its own split product
+        groups = mgx.summa_groups(code, panel, k)
+        assert groups == [1024, 1024, 1024, 1024, 904]
+        assert st["groups"] == 5
+        ref = _grouped(A, B, groups, lambda a, b, C: orc.crt_gemm(a, b, 15, C=C))
         assert np.array_equal(C, ref)
 
 
@@ -153,14 +155,19 @@ def test_graph_replays_are_identical(mgx, orc):
         assert np.array_equal(first, again)
 
 
-def _grouped(A, B, g, product):
-    """The oracle's split product per group of g columns, accumulated into C
-    in group order."""
+def _grouped(A, B, groups, product):
+    """The oracle's split product per group (uniform groups of `groups`
+    columns, or the listed widths), accumulated into C in group order."""
     k = A.shape[1]
+    if isinstance(groups, int):
+        groups = [min(groups, k - g0) for g0 in range(0, k, groups)]
+    assert sum(groups) == k
     ref = np.zeros((A.shape[0], B.shape[1]))
-    for g0 in range(0, k, g):
-        ref = product(np.ascontiguousarray(A[:, g0:g0 + g]), np.ascontiguousarray(B[g0:g0 + g]),
+    g0 = 0
+    for w in groups:
+        ref = product(np.ascontiguousarray(A[:, g0:g0 + w]), np.ascontiguousarray(B[g0:g0 + w]),
                       C=ref)
+        g0 += w
     return ref
 
 
@@ -174,8 +181,10 @@ def test_split_forms_multiply_whole_panel_groups(mgx, orc, form_name, split_k, g
     announce(f"summa {form_name} split_k={split_k}")
     form = {"int8": mgx.FORM_INT8, "mod": mgx.FORM_INT8_MOD, "bf16": mgx.FORM_BF16}[form_name]
     assert mgx.split_group_k(form, panel, k, split_k) == group
-    # FP64 groups whole panels by the same rule (default 2048 columns)
+    # FP64 groups whole panels by the same rule (default 1024 columns)
     assert mgx.split_group_k(mgx.FORM_FP64, panel, k, split_k) == group
+    uniform = [min(group, k - g0) for g0 in range(0, k, group)]
+    assert mgx.summa_groups(form, panel, k, split_k) == uniform
     A = orc.fill(m, k, 1, seed=41)
     B = orc.fill(k, n, 1, seed=42)
     if form_name in ("int8", "bf16"):
@@ -247,6 +256,10 @@ def test_invalid_summa_arguments(mgx):
                       use_graph=True)
     with pytest.raises(mgx.MgxError):
         mgx.summa_run(A.ctypes.data, 4, A.ctypes.data, 4, dC, 4, m=4, n=4, k=4, panel=0)
+    for threads in (-1, 65):
+        with pytest.raises(mgx.MgxError):
+            mgx.summa_run(A.ctypes.data, 4, A.ctypes.data, 4, dC, 4, m=4, n=4, k=4, panel=2,
+                          host_threads=threads)
 
 
 # ---- accuracy report -------------------------------------------------------------------
@@ -284,3 +297,55 @@ def test_error_report_planted_error(mgx, orc):
     C[5, 6] = np.nan
     rep = mgx.error_report(to_dev(A), to_dev(B), to_dev(C), m, n, k, k, n, n, np.arange(m))
     assert np.isinf(rep["normwise"])
+
+
+@pytest.mark.parametrize("form_name", ["mod", "int8"])
+def test_tapered_schedule(mgx, orc, form_name):
+    # 24 panels of 256 (ragged last of 112), tapered schedule on groups of 16
+    # panels: one group of 8 panels, then four of 4; each launch is one split
+    # product, bit-exact against the oracle's model at its own depth; pinned,
+    # graph and pageable runs agree
+    m, n, k, panel = 96, 80, 6000, 256
+    announce(f"summa {form_name} tapered schedule")
+    form, sl = ((mgx.FORM_INT8_MOD, 15) if form_name == "mod" else (mgx.FORM_INT8, 8))
+    groups = mgx.summa_groups(form, panel, k, -4096)
+    assert groups == [2048, 1024, 1024, 1024, 880]
+    A = orc.fill(m, k, 1, seed=61)
+    B = orc.fill(k, n, 1, seed=62)
+    C, st = _run(mgx, A, B, panel, pinned=True, form=form, slices=sl, split_k=-4096)
+    assert st["groups"] == 5
+    if form_name == "mod":
+        ref = _grouped(A, B, groups, lambda a, b, C: orc.crt_gemm(a, b, 15, C=C))
+    else:
+        ref = _grouped(A, B, groups, lambda a, b, C: orc.split_gemm_int8(a, b, 8, C=C))
+    assert np.array_equal(C, ref)
+    C_graph, _ = _run(mgx, A, B, panel, pinned=True, graph=True, form=form, slices=sl,
+                      split_k=-4096)
+    C_page, _ = _run(mgx, A, B, panel, pinned=False, form=form, slices=sl, split_k=-4096)
+    assert np.array_equal(C, C_graph) and np.array_equal(C, C_page)
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_pageable_packing_threads(mgx, orc, threads):
+    # pageable strips packed by 1, 3 or 8 host threads into staging units of
+    # four panels (ragged last panel, units straddling no group boundary):
+    # FP64 integer inputs exact, modular groups equal to the pinned run
+    r = np.random.default_rng(5 + threads)
+    m, n, k, panel = 300, 260, 3000, 256
+    announce(f"summa pageable packing threads={threads}")
+    A = r.integers(-60, 60, size=(m, k)).astype(np.float64)
+    B = r.integers(-60, 60, size=(k, n)).astype(np.float64)
+    dC = empty(m, n, fill=0.0)
+    st = mgx.summa_run(A.ctypes.data, k, B.ctypes.data, n, dC, n, m=m, n=n, k=k, panel=panel,
+                       host_threads=threads)
+    assert st["host_pinned"] == 0 and st["host_threads"] == threads
+    assert st["groups"] == len(mgx.summa_groups(mgx.FORM_FP64, panel, k))
+    exact = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float64)
+    assert np.array_equal(to_host(dC), exact)
+    Ar = orc.fill(m, k, 1, seed=70 + threads)
+    Br = orc.fill(k, n, 1, seed=80 + threads)
+    C_pin, _ = _run(mgx, Ar, Br, panel, pinned=True, form=mgx.FORM_INT8_MOD, slices=15)
+    dC = empty(m, n, fill=0.0)
+    mgx.summa_run(Ar.ctypes.data, k, Br.ctypes.data, n, dC, n, m=m, n=n, k=k, panel=panel,
+                  form=mgx.FORM_INT8_MOD, slices=15, host_threads=threads)
+    assert np.array_equal(to_host(dC), C_pin)

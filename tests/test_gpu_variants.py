@@ -80,8 +80,13 @@ def test_autotune_cycle_small(mgx, tmp_path):
               (768, 512, 1536), (512, 768, 1024)]
     doc = at.measure(shapes, tmp_path / "ds.json", log=lambda *_: None)
     rows = doc["rows"]
-    assert len(rows) == sum(len(at.candidate_space(k)) for _, _, k in shapes)
+    assert len(rows) == sum(len(at.candidate_space(k, t)) for _, _, k in shapes
+                            for t in at.TARGETS)
     assert all(r["ms"] > 0 for r in rows)
+    # resident split products do not depend on the panel width: timed once
+    res_mod = {r["ms"] for r in rows if r["target"] == "resident" and r["form"] == "mod"
+               and r["slices"] == 15 and r["tile"] == 1 and (r["m"], r["n"], r["k"]) == shapes[0]}
+    assert len(res_mod) == 1
     # resume: nothing left to measure
     again = at.measure(shapes, tmp_path / "ds.json", log=lambda *_: None)
     assert len(again["rows"]) == len(rows)
@@ -89,11 +94,16 @@ def test_autotune_cycle_small(mgx, tmp_path):
     model, hist = at.train(rows, tr, epochs=600)
     assert hist[-1] < hist[0]
     res = at.evaluate(rows, model, tr, ho)
-    assert res["heldout_shapes"] == 2 and res["mape"] is not None
+    for t in at.TARGETS:
+        assert res["targets"][t]["heldout_shapes"] == 2
+        assert res["targets"][t]["mape"] is not None
+        assert res["targets"][t]["fixed_candidate"] is not None
     timer = at.Timer(1024, 1024, 2048)
-    choice, ranked = at.select(model, 512, 512, 1024, topk=3, timer=timer)
-    assert "measured_ms" in choice and len(ranked) == len(at.candidate_space(1024))
+    for target in at.TARGETS:
+        choice, ranked = at.select(model, 512, 512, 1024, target, topk=3, timer=timer)
+        assert "measured_ms" in choice and len(ranked) == len(at.candidate_space(1024, target))
     # the model round-trips through JSON
     m2 = at.MLPModel.from_json(model.to_json())
-    X = np.array([at.features(512, 512, 1024, c) for c in at.candidate_space(1024)])
+    X = np.array([at.features(512, 512, 1024, c, "streamed")
+                  for c in at.candidate_space(1024, "streamed")])
     assert np.allclose(m2.predict_log_ms(X), model.predict_log_ms(X))
