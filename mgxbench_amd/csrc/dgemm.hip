@@ -21,9 +21,12 @@
 // launch however many panels the launch covers.
 //
 // Design (gfx950):
-//  * Tile: 128 x 128 C block per workgroup of 8 wave64s (2 x 4), each wave
-//    64 x 32 as 4 x 2 MFMA fragments; one workgroup per CU, two waves per
-//    SIMD.  Small variant: 64 x 64 with 4 waves of 32 x 32, two per CU.
+//  * Tiles: panel-fused, 128 x 128 C block per workgroup of 8 wave64s
+//    (2 x 4), each wave 64 x 32 as 4 x 2 MFMA fragments (two register sets:
+//    running sum and partial); one workgroup per CU, two waves per SIMD.
+//    Small variant: 64 x 64 with 4 waves of 32 x 32, two per CU.  A
+//    one-panel product (the partial is the result) runs a 256 x 128 block,
+//    8 waves of 64 x 64, with a 3-deep ring (fewer loads per MFMA).
 //  * Staging: a ring of NSTAGE LDS slots, each the A and B tiles of one
 //    16-deep k stage (32 KiB at 128 x 128), filled by direct global->LDS loads
 //    (global_load_lds_dwordx4: lane l writes its 16 bytes at l * 16 of a 1 KiB
@@ -37,6 +40,11 @@
 //    takes k = 8P + 2g + h for step h of pair P.  B is read with the same k
 //    assignment; the 8 lanes of each read pass touch 8 different chunk
 //    positions, so neither operand has bank conflicts.
+//  * Issue order: each pair of k steps issues MFMA_LEAD MFMAs, then the
+//    fragment reads of the next pair, then its other MFMAs, so the reads
+//    overlap matrix work without a wait in front of the first MFMA; the
+//    B fragments of a pair are separate registers per step, which keeps the
+//    compiler from inserting copies.
 //  * Edges: rows of A past M and columns of B past N are clamped to valid
 //    addresses and their outputs discarded.  A stage that crosses the end of
 //    its panel, and every stage of operands that are not 16-byte aligned with
@@ -121,23 +129,14 @@ struct DShape {
     static_assert(LDS_BYTES <= 160 * 1024, "LDS budget");
     static_assert((NSTAGE - 2) * LPS < 64 && NSTAGE >= 2, "ring depth");
 };
-#ifndef MGX_DG_KS
-#define MGX_DG_KS 16
-#endif
-#ifndef MGX_DG_NSTAGE
-#define MGX_DG_NSTAGE 4
-#endif
-#ifndef MGX_DG_MONO
-#define MGX_DG_MONO 2
-#endif
-#ifndef MGX_DG_SPLIT
-#define MGX_DG_SPLIT 2
-#endif
-#ifndef MGX_DG_LEAD
-#define MGX_DG_LEAD 2
-#endif
+// MFMAs of a pair of k steps issued before the next pair's fragment reads
+// (so no LDS read stands in front of the matrix core), and MFMAs of a
+// stage's last pair issued between the stage's publish wait and its next
+// load issue.
+constexpr int MFMA_LEAD = 2;
+constexpr int MFMA_BEFORE_ISSUE = 2;
 // panel-fused tiles (running sum + partial in registers)
-using Large = DShape<128, 128, 2, 4, MGX_DG_KS, MGX_DG_NSTAGE>;
+using Large = DShape<128, 128, 2, 4, 16, 4>;
 using Small = DShape<64, 64, 2, 2, 16, 4>;
 // one-panel tile (the partial is the result): 256 x 128, 8 waves of 64 x 64
 using Wide = DShape<256, 128, 4, 2, 16, 3>;
@@ -380,8 +379,8 @@ __global__ __launch_bounds__(S::THREADS, 1) void dgemm_panels(PanelArgs a) {
                 }
             };
             using Q0 = std::integral_constant<int, 0>;
-            using QH = std::integral_constant<int, MGX_DG_SPLIT>;
-            using QA = std::integral_constant<int, MGX_DG_LEAD>;
+            using QH = std::integral_constant<int, MFMA_BEFORE_ISSUE>;
+            using QA = std::integral_constant<int, MFMA_LEAD>;
             using QE = std::integral_constant<int, 2 * FI * FJ>;
             StageCursor<KS> cs;
             cs.start((int)a.K, (int)a.panel);
@@ -524,12 +523,10 @@ int launch_dgemm(int tile, bool accumulate, const double *A, const double *B, do
     const bool one_panel = panel >= k;
     switch (tile) {
         case MGX_TILE_LARGE:
-            if (one_panel && MGX_DG_MONO == 2)
+            // a one-panel product needs no running sum: the wider tile
+            if (one_panel)
                 return launch_panels<Wide, false>(A, B, C, m, n, k, panel, lda, ldb, ldc,
                                                   accumulate, st);
-            if (one_panel && MGX_DG_MONO == 1)
-                return launch_panels<Large, false>(A, B, C, m, n, k, panel, lda, ldb, ldc,
-                                                   accumulate, st);
             return launch_panels<Large, true>(A, B, C, m, n, k, panel, lda, ldb, ldc, accumulate,
                                               st);
         case MGX_TILE_SMALL:

@@ -379,7 +379,7 @@ ORC_API int orc_split_gemm_int8(int s, const double *A, const double *B, double 
 #pragma omp parallel for schedule(dynamic, 1) collapse(2) reduction(| : overflow)
     for (int64_t i = 0; i < m; ++i) {
         for (int64_t j = 0; j < n; ++j) {
-            /* Fold sequence of the product kernel: x = ldexp(G, e_i+f_j-7t) per
+            /* Fold sequence of the product kernel: x = ldexp(G, e_i+f_j+4-8t) per
              * (group, chunk); C = x (or C_in + x) first, then C = C + x. */
             double v = accumulate ? C[i * ldc + j] : 0.0;
             int first = 1;
