@@ -60,7 +60,10 @@ import numpy as np
 
 from . import _lib
 
-SIZES = (1024, 2048, 3072, 4096, 6144, 8192, 12288, 16384)
+# shape sizes: 4096 ... 32768 in steps of 4096 (the rank shares of the 1x4 /
+# 2x2 / 4x1 grids up to N = 32768) plus smaller ones, where the 128 tile and
+# the plain FP64 form win
+SIZES = (1024, 2048, 3072, 4096, 6144, 8192, 12288, 16384, 20480, 24576, 28672, 32768)
 PANELS = (128, 256, 512, 1024)
 SLICES = (8,)
 MODULI = (15, 16)
@@ -272,10 +275,11 @@ class Timer:
         if (c.form, c.tile) not in self._warm:     # first use loads the kernels
             self.mgx.summa_run(self.hA.data_ptr(), k, self.hB.data_ptr(), n, self.C, n, **kw)
             self._warm.add((c.form, c.tile))
-        # the faster of two loops: a loop is one pass over the strips, and a
-        # single pass varies by a few percent
+        # the faster of two loops (a short loop varies by a few percent); a
+        # loop copying for more than 60 ms runs once
+        loops = 2 if 8.0 * (m * k + k * n) / (PCIE_GBPS * 1e6) < 60.0 else 1
         return min(self.mgx.summa_run(self.hA.data_ptr(), k, self.hB.data_ptr(), n, self.C, n,
-                                      **kw)["seconds"] for _ in range(2)) * 1e3
+                                      **kw)["seconds"] for _ in range(loops)) * 1e3
 
     def time(self, m: int, n: int, k: int, c: Candidate, target: str = "resident",
              est_ms: float | None = None) -> float:
@@ -606,7 +610,7 @@ def main(argv=None) -> int:
     sub = ap.add_subparsers(dest="cmd", required=True)
     p = sub.add_parser("measure")
     p.add_argument("--out", required=True)
-    p.add_argument("--random-shapes", type=int, default=56)
+    p.add_argument("--random-shapes", type=int, default=84)
     p.add_argument("--max-size", type=int, default=max(SIZES))
     p.add_argument("--targets", nargs="+", choices=TARGETS, default=list(TARGETS))
     p.add_argument("--time-budget", type=float, default=None,

@@ -60,7 +60,7 @@ def test_step_depth_is_the_whole_k_or_a_whole_panel_group():
 def test_rank_shares_are_the_grid_shares():
     shapes = at.rank_shares([16384])
     assert shapes == [(16384, 4096, 16384), (8192, 8192, 16384), (4096, 16384, 16384)]
-    assert len(at.rank_shares()) == 24
+    assert len(at.rank_shares()) == 36
     r = at.random_shapes(10)
     assert len(set(r)) == 10 and r == at.random_shapes(10)
 
@@ -93,7 +93,7 @@ def test_shape_split_is_deterministic_and_disjoint():
     tr, ho = at.shape_split(shapes, 0.25)
     assert tr == at.shape_split(shapes, 0.25)[0]
     assert not set(tr) & set(ho) and len(tr) + len(ho) == len(shapes)
-    assert len(ho) == 16
+    assert len(ho) == round(0.25 * len(shapes))
 
 
 def _synthetic_rows(shapes):
@@ -189,11 +189,11 @@ def test_share_shape():
 
 def test_random_shapes_exclusion_for_heldout_sets():
     from mgxbench_amd import autotune as at
-    train = at.rank_shares() + at.random_shapes(56)
+    train = at.rank_shares() + at.random_shapes(84)
     held = at.random_shapes(200, seed=8, exclude=train)
     assert len(set(held)) == 200 and not set(held) & set(train)
     assert held == at.random_shapes(200, seed=8, exclude=train)       # deterministic
-    assert at.random_shapes(56) == train[24:]                          # default stream unchanged
+    assert at.random_shapes(84) == train[36:]                          # default stream unchanged
     space = len(at.SIZES) ** 3
     with pytest.raises(ValueError):                                    # one shape of the space excluded
         at.random_shapes(space, exclude=at.random_shapes(1))
