@@ -32,6 +32,9 @@
 
 using namespace mgxdrv;
 
+// Default columns per split product in the K-chunk pipeline.
+constexpr int64_t PIPELINE_SPLIT_K = 4096;
+
 #define TILE_DIM 32
 
 static void fail(const char *msg, int code = 1) {
@@ -118,12 +121,15 @@ int main(int argc, char **argv) {
     if (warmup_count < 0) fail("MGXBENCH_INTERNAL_WARMUP must be non-negative", 4);
 
     const bool pipelined = libB == 'S' || libB == 'Y' || libB == 'X';
-    // split forms stage whole groups of chunks per split product
-    // (MGX_SPLIT_K columns, default MGX_SPLIT_GROUP_K)
+    // split forms stage whole groups of chunks per split product (MGX_SPLIT_K
+    // columns, default PIPELINE_SPLIT_K): the chunk copies here are short
+    // device-side copies, so the products set the pace and wide groups
+    // amortise a split product's fixed costs (exponent scan, whole-C
+    // reconstruction) over more depth
     const int kchunk = opt.form == MGX_FORM_FP64
                            ? pipeline_k_chunk((int)k)
                            : (int)mgx_split_group_k(opt.form, pipeline_k_chunk((int)k),
-                                                    env_int("MGX_SPLIT_K", 0), k);
+                                                    env_int("MGX_SPLIT_K", PIPELINE_SPLIT_K), k);
     hipStream_t comm = nullptr, compute = nullptr;
     hipEvent_t b_ready[2] = {}, compute_done[2] = {};
     double *b_stage[2] = {nullptr, nullptr};
