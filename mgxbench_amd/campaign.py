@@ -116,13 +116,6 @@ def plan(decompositions, sizes, ranks, mode, base, alignment=4, rows=None, cols=
     return points
 
 
-def configurations(decompositions, sizes, ranks, mode, base, alignment=4, rows=None, cols=None,
-                   log=lambda s: print(s, file=sys.stderr)) -> list[tuple[str, int, int]]:
-    """The campaign's points as (decomposition, N, ranks)."""
-    return [pt.as_tuple() for pt in plan(decompositions, sizes, ranks, mode, base, alignment,
-                                         rows, cols, log)]
-
-
 def write_manifest(path: Path, points: list[Point], mode: str, base: int, alignment: int) -> None:
     with path.open("w", newline="", encoding="utf-8") as f:
         out = csv.DictWriter(f, fieldnames=MANIFEST_FIELDS)

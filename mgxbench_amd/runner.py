@@ -362,8 +362,8 @@ def main(argv: list[str] | None = None) -> int:
     args = ap.parse_args(argv)
     grid = validate_args(ap, args)
     if args.libraries == ["AUTO"]:
-        from .predictive import auto
-        return auto(args)
+        from .predictive import run_auto
+        return run_auto(args)
     bad = [lib for lib in args.libraries if not re.fullmatch("[MYCXNSW]{3}", lib)]
     if bad:
         ap.error(f"not a three-letter library code: {bad[0]}")

@@ -24,22 +24,24 @@ def test_weak_scaling_sizes():
         campaign.weak_scaling_size(0, 2)
 
 
-def test_configurations_rules():
+def _points(*args, **kw):
+    return [pt.as_tuple() for pt in campaign.plan(*args, **kw)]
+
+
+def test_plan_rules():
     logs = []
-    cfg = campaign.configurations(["1d"], [4096, 4098], [1, 4], "sweep", 4096,
-                                  log=logs.append)
+    cfg = _points(["1d"], [4096, 4098], [1, 4], "sweep", 4096, log=logs.append)
     assert cfg == [("1d", 4096, 1), ("1d", 4096, 4)]
     assert any("4098" in s for s in logs)
-    weak = campaign.configurations(["1d", "2d"], [], [1, 2, 4, 8], "weak", 4096)
+    weak = _points(["1d", "2d"], [], [1, 2, 4, 8], "weak", 4096)
     for d, n, p in weak:
         assert abs(n ** 3 / p / 4096 ** 3 - 1) <= 0.025
     with pytest.raises(ValueError):                      # rounding moves N^3/p by 12.5 %
-        campaign.configurations(["1d"], [], [3], "weak", 8)
+        _points(["1d"], [], [3], "weak", 8)
     logs.clear()
     # fixed 2 grid rows: p = 4 -> 2x2 (multiple of 4), p = 6 -> 2x3 (multiple of 12:
     # 4096 is not), p = 3 has no grid with 2 rows
-    cfg = campaign.configurations(["2d"], [4096], [4, 6, 3], "sweep", 4096, rows=2,
-                                  log=logs.append)
+    cfg = _points(["2d"], [4096], [4, 6, 3], "sweep", 4096, rows=2, log=logs.append)
     assert cfg == [("2d", 4096, 4)]
     assert any("p=3" in s for s in logs) and any("p=6" in s for s in logs)
 
