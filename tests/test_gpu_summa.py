@@ -380,3 +380,60 @@ def test_copies_overlap_products(mgx, form_name):
     print(f"{form_name}: overlapped {times[False] * 1e3:.2f} ms, serialized "
           f"{times[True] * 1e3:.2f} ms", flush=True)
     assert times[False] < 0.9 * times[True]
+
+
+@pytest.mark.parametrize("panel", [256, 512, 1024])
+def test_share_2x2_n16384_streamed(mgx, panel):
+    """The 2x2 rank share at N = 16384 (C 8192 x 8192, K = 16384) as the
+    streamed loop, in panels of b = 256, 512 and 1024 from pinned host strips
+    (1 GiB each).  FP64: bit-identical to the resident panel-fused launch over
+    the same panels (one partial per panel, added in order).  Modular form,
+    15 moduli: bit-identical to split products of the loop's panel groups
+    accumulated on the device in order, and no worse than plain FP64 against
+    the double-double product on sampled rows."""
+    import torch
+    m = n = 8192
+    k = 16384
+    announce(f"summa 2x2 share N=16384 streamed b={panel}")
+    dev = torch.device("cuda", 0)
+    dA = torch.empty((m, k), dtype=torch.float64, device=dev)
+    dB = torch.empty((k, n), dtype=torch.float64, device=dev)
+    mgx.fill(dA, m, k, k, 1, seed=21)
+    mgx.fill(dB, k, n, n, 1, seed=22)
+    pa, pb = PinnedArray(mgx, m, k), PinnedArray(mgx, k, n)
+    try:
+        torch.from_numpy(pa.array).copy_(dA)
+        torch.from_numpy(pb.array).copy_(dB)
+        rows = np.arange(0, m, 1024)
+        out = {}
+        for form, sl in ((mgx.FORM_FP64, 8), (mgx.FORM_INT8_MOD, 15)):
+            dC = torch.zeros((m, n), dtype=torch.float64, device=dev)
+            st = mgx.summa_run(pa.ptr, k, pb.ptr, n, dC, n, m=m, n=n, k=k, panel=panel,
+                               form=form, slices=sl)
+            assert st["panels"] == k // panel and st["host_pinned"] == 1
+            out[form] = dC
+    finally:
+        pa.free()
+        pb.free()
+    ref = torch.empty((m, n), dtype=torch.float64, device=dev)
+    mgx._lib.call("mgx_dgemm_panels", 0, 0, dA.data_ptr(), dB.data_ptr(), ref.data_ptr(),
+                  m, n, k, panel, k, n, n, 0)
+    assert torch.equal(out[mgx.FORM_FP64], ref)
+    ref.zero_()
+    k0 = 0
+    for g in mgx.summa_groups(mgx.FORM_INT8_MOD, panel, k):
+        mgx._lib.call("mgx_split_gemm_ex", mgx.FORM_INT8_MOD, 15, 0, dA.data_ptr() + 8 * k0,
+                      dB.data_ptr() + 8 * k0 * n, ref.data_ptr(), m, n, g, k, n, n, 1, None,
+                      0, 0)
+        k0 += g
+    assert k0 == k
+    assert torch.equal(out[mgx.FORM_INT8_MOD], ref)
+    plain = mgx.error_report(dA, dB, out[mgx.FORM_FP64], m, n, k, k, n, n, rows)
+    mod = mgx.error_report(dA, dB, out[mgx.FORM_INT8_MOD], m, n, k, k, n, n, rows)
+    print(f"b={panel}: normwise plain {plain['normwise']:.3e} modular {mod['normwise']:.3e}",
+          flush=True)
+    assert plain["normwise"] < 1e-14
+    assert mod["normwise"] <= plain["normwise"]
+    assert mod["componentwise"] <= plain["componentwise"]
+    del dA, dB, out, ref
+    torch.cuda.empty_cache()
