@@ -142,8 +142,10 @@ int mgx_validate_constant(const double *dC, int64_t rows, int64_t cols,
  * Size limits (grid dimensions of the operand passes): m and k at most
  * 4194048; larger calls return MGX_ERR_INVALID_ARG before any launch.
  * MGX_FORM_INT8_MOD: `slices` is the number of moduli (2..16); the result
- * is ldexp(RN(A'B'), e_i + f_j - 2 beta) with A', B' the 2^beta-scaled
- * integer operands (see mgx_split_mod_beta), so each output is rounded once.
+ * is ldexp(RN(A'B'), e_i - beta_i + f_j - delta_j) with A', B' the integer
+ * operands, row i of A scaled by 2^(beta_i - e_i) and column j of B by
+ * 2^(delta_j - f_j) (per-line exponents, see mgx_split_mod_beta), so each
+ * output is rounded once.
  * Its events: [2] before the residue products, [3] after them (the
  * reconstruction into C follows [3]).
  * MGX_FORM_BF16: `slices` 1..9 as for MGX_FORM_INT8, same digits, group
@@ -201,10 +203,16 @@ int mgx_split_gemm_noalloc(int form, int slices,
  * MGX_FORM_INT8_MOD the count is the number of moduli. */
 int mgx_split_slices_range(int form, int *min_slices, int *max_slices);
 
-/* Scaling exponent beta of MGX_FORM_INT8_MOD for `moduli` moduli and depth k:
- * A and B are quantised to beta-bit integers per row / column (the form's
- * only error); the largest beta <= 53 with 2 k 4^beta < m_0 ... m_{n-1}.
- * Returns -1 when the moduli cannot hold the product (the call would fail). */
+/* Guaranteed scaling exponent of MGX_FORM_INT8_MOD for `moduli` moduli and
+ * depth k.  Each row of A and column of B is quantised to an integer line of
+ * at most 2^beta (the form's only error), with beta chosen per line from an
+ * exponent histogram of the line so that a Cauchy-Schwarz bound keeps every
+ * |A'B'| entry below m_0 ... m_{n-1} / 2; lines spread over many binades get
+ * more bits.  This returns the exponent a line gets when all its elements
+ * sit in its top binade, which every line gets at least (the classical
+ * largest beta <= 53 with 2 k 4^beta < m_0 ... m_{n-1}, up to a rounding
+ * margin).  Returns -1 when the moduli cannot hold the product (the call
+ * would fail). */
 int mgx_split_mod_beta(int moduli, int64_t k);
 
 /* ---- accuracy report ----------------------------------------------------

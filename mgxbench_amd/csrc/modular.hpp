@@ -4,12 +4,14 @@
 //
 // Published algorithm: K. Ozaki, Y. Uchino, T. Imamura, "Ozaki Scheme II: A
 // GEMM-oriented emulation of floating-point matrix multiplication using an
-// integer modular technique" (2025).  A row of A is scaled to integers
-// A' = rint(A 2^(beta-e_i)), |A'| <= 2^beta, B per column likewise; for each
-// modulus m_l the INT8 matrix cores form (A' mod m_l)(B' mod m_l) exactly in
-// INT32, reduced mod m_l; Chinese remaindering over the n moduli recovers the
-// integer C' = A'B' exactly because 2 k 4^beta < P = m_0 ... m_{n-1}.  The
-// output is ldexp(RN(C'), e_i + f_j - 2 beta): one rounding.
+// integer modular technique" (2025).  Row i of A is scaled to integers
+// A' = rint(A 2^(beta_i-e_i)), |A'| <= 2^beta_i, column j of B likewise with
+// its own delta_j; for each modulus m_l the INT8 matrix cores form
+// (A' mod m_l)(B' mod m_l) exactly in INT32, reduced mod m_l; Chinese
+// remaindering over the n moduli recovers the integer C' = A'B' exactly
+// because every |C'_ij| < P/2, P = m_0 ... m_{n-1} (the per-line scaling rule
+// below guarantees it).  The output is ldexp(RN(C'), e_i - beta_i + f_j -
+// delta_j): one rounding.
 #pragma once
 
 #include <stdint.h>
@@ -40,20 +42,105 @@ constexpr uint32_t inverse_mod(uint32_t a, uint32_t m) {
     return (uint32_t)(((t0 % (int64_t)m) + m) % m);
 }
 
-// Scaling exponent for n moduli and depth k: the largest beta in [0, 53] with
-// 2 k 4^beta < P_n, or -1 when even beta = 0 does not fit.
-inline int beta_for(int n, int64_t k) {
-    if (n < MIN_MODULI || n > MAX_MODULI || k < 1) return -1;
+// ---- per-line scaling exponents ------------------------------------------
+//
+// By Cauchy-Schwarz |C'_ij| <= ||A'_i||_2 ||B'_j||_2, so each line (a row of
+// A, a column of B) gets its own scale 2^(beta - E), E = the frexp exponent of
+// the line's largest |x|, with beta as large as keeps ||X'||_2 below a radius
+// R where (R + sqrt(k)/2)^2 < P/2 (sqrt(k)/2 bounds the rounding of the
+// scaled line to integers).  The 2-norm is bounded from exponents alone, so
+// the bound is an exact integer that no summation order changes: with e(x)
+// the frexp exponent of x (|x| < 2^e(x)),
+//   N = sum over groups of GROUP_K consecutive k, over the nonzero x of the
+//       group, of 4^(W - min(g - e(x), W)),   g = the group's largest e(x),
+// and sum |x|^2 < 4^E N / 4^W, because every term relative to the group's
+// maximum is at least the term relative to the line's (E >= g), and an
+// element more than W binades below its group's maximum still counts 4^0.
+// Then ||X 2^(beta - E)||_2 < 2^beta sqrt(N / 4^W) <= R whenever
+// 4^beta N <= R^2 4^W, and beta is the largest such value in [0, 53].  A line
+// whose elements all sit in its top binade has N = k 4^W and gets the
+// classical worst-case exponent (largest beta with 2 k 4^beta < P, up to the
+// rounding margin); lines with a spread of magnitudes get more bits.
+constexpr int HIST_W = 12;       // binades resolved below a group maximum
+constexpr int GROUP_K = 16;      // k values per group
+constexpr int BETA_MAX = 53;
+
+// Largest N that allows beta = b, for b = 0..53 (non-increasing in b).
+struct BetaThresholds {
+    unsigned long long t[BETA_MAX + 1];
+};
+
+inline unsigned __int128 isqrt_u128(unsigned __int128 x) {
+    // floor(sqrt(x)) by bisection on the result's bits
+    unsigned __int128 r = 0;
+    for (int b = 63; b >= 0; --b) {
+        const unsigned __int128 c = r | ((unsigned __int128)1 << b);
+        if (c * c <= x) r = c;
+    }
+    return r;
+}
+
+// R^2 for n moduli at depth k: R = floor(sqrt(P/2)) - ceil(ceil(sqrt(k)) / 2) - 1,
+// so R + sqrt(k)/2 < sqrt(P/2); 0 when R < 1 or the arguments are out of range.
+inline unsigned __int128 radius2(int n, int64_t k) {
+    if (n < MIN_MODULI || n > MAX_MODULI || k < 1) return 0;
     unsigned __int128 P = 1;
     for (int l = 0; l < n; ++l) P *= MODULI[l];
-    unsigned __int128 lhs = (unsigned __int128)2 * (uint64_t)k;
-    if (lhs >= P) return -1;
-    int beta = 0;
-    while (beta < 53 && lhs <= (P - 1) / 4 && lhs * 4 < P) {
-        lhs *= 4;
-        ++beta;
+    const unsigned __int128 s = isqrt_u128(P / 2);
+    unsigned __int128 h = isqrt_u128((unsigned __int128)k);
+    if (h * h < (unsigned __int128)k) ++h;
+    const unsigned __int128 cut = (h + 1) / 2 + 1;
+    if (s <= cut) return 0;
+    const unsigned __int128 R = s - cut;
+    return R * R;
+}
+
+// Thresholds of the per-line rule: beta = b is allowed iff N <= t[b], with
+// t[b] = floor(R^2 4^(W - b)) saturated to 2^64 - 1.  False when no R exists.
+inline bool beta_thresholds(int n, int64_t k, BetaThresholds *T) {
+    const unsigned __int128 R2 = radius2(n, k);
+    if (R2 == 0) return false;
+    const unsigned __int128 cap = ~(unsigned long long)0;
+    for (int b = 0; b <= BETA_MAX; ++b) {
+        unsigned __int128 v;
+        if (b >= HIST_W) {
+            const int sh = 2 * (b - HIST_W);
+            v = sh >= 128 ? 0 : R2 >> sh;
+        } else {
+            const int sh = 2 * (HIST_W - b);
+            v = R2 > (cap >> sh) ? cap : R2 << sh;
+        }
+        T->t[b] = (unsigned long long)(v > cap ? cap : v);
     }
-    return beta;
+    return true;
+}
+
+// The largest beta in [0, 53] with N <= t[beta], or -1 when N exceeds t[0].
+__host__ __device__ inline int line_beta(const BetaThresholds &T, unsigned long long N) {
+    if (N > T.t[0]) return -1;
+    int lo = 0, hi = BETA_MAX;   // t[lo] >= N
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (N <= T.t[mid]) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Group term 4^(W - min(g - e, W)) of one element with exponent e in a group
+// whose largest exponent is g >= e.
+__host__ __device__ inline unsigned long long group_term(int g, int e) {
+    const int d = g - e < HIST_W ? g - e : HIST_W;
+    return 1ull << (2 * (HIST_W - d));
+}
+
+// The guaranteed exponent for n moduli at depth k: the beta of a line whose
+// elements all sit in its top binade (N = k 4^W), which every line gets at
+// least; -1 when the moduli cannot hold even that line at beta = 0.
+inline int beta_for(int n, int64_t k) {
+    BetaThresholds T;
+    if (k < 1 || k > ((int64_t)1 << 36) || !beta_thresholds(n, k, &T)) return -1;
+    return line_beta(T, (unsigned long long)k << (2 * HIST_W));
 }
 
 // Residue of an integer-valued double |x| <= 2^53 modulo MODULI[l], as a

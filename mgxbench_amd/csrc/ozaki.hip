@@ -26,19 +26,20 @@
 //     C = x (or C_in + x for the accumulate form), later folds C = C + x.
 //
 // Modular form (MGX_FORM_INT8_MOD, Ozaki scheme II; modular.hpp):
-//  1. A'_ik = rint(A_ik 2^(beta-e_i)) (|A'| <= 2^beta), B' likewise, with
-//     beta the largest exponent <= 53 such that 2 k 4^beta < P, the product
-//     of the first n moduli.  Operands are the residues of A', B' modulo
-//     each m_l, as int8.
+//  1. A'_ik = rint(A_ik 2^(beta_i-e_i)) (|A'| <= 2^beta_i), B' likewise per
+//     column with delta_j; each line's exponent comes from an exponent-group
+//     sum of the line by a Cauchy-Schwarz bound that keeps every |C'_ij|
+//     below P/2, P the product of the first n moduli (modular.hpp).
+//     Operands are the residues of A', B' modulo each m_l, as int8.
 //  2. For each modulus the INT8 MFMA forms the residue product exactly in
 //     INT32 (reduced mod m_l between K chunks of at most 2046 tiles) and
 //     stores it mod m_l as one byte per output.
 //  3. A reconstruction kernel recovers the integer C' = A'B' from its n
 //     residues (direct Chinese remaindering in FP64 on exact 40-bit limbs),
-//     rounds it once to FP64 and writes C = ldexp(C', e_i + f_j - 2 beta)
-//     (or C_in + that).
+//     rounds it once to FP64 and writes
+//     C = ldexp(C', e_i - beta_i + f_j - delta_j) (or C_in + that).
 // The modular form needs n products instead of s(s+1)/2 and its only error
-// is the quantisation of A and B to beta bits.
+// is the quantisation of each line of A and B to its beta bits.
 #include "mgx_internal.hpp"
 #include "modular.hpp"
 
@@ -80,61 +81,176 @@ __device__ inline unsigned long long abs_bits(double x) {
     return (unsigned long long)__double_as_longlong(x) & 0x7fffffffffffffffull;
 }
 
-// Row maxima of A (m x k, lda): grid.x covers k in segments of 256*16,
-// grid.y strides over rows.  Maxima are kept as the bit patterns of |x|,
-// which order like the values for non-negative doubles (NaN above Inf).
-__global__ __launch_bounds__(256) void row_absmax_kernel(const double *__restrict__ A, int64_t m,
-                                                         int64_t k, int64_t lda,
-                                                         unsigned long long *rowmax) {
-    __shared__ unsigned long long sh[4];
+// Per-element exponent data: e = the frexp exponent of a finite nonzero x
+// (|x| < 2^e, subnormals included; v_frexp_exp_i32_f64) or NO_EXP, and the
+// line-maximum code: 0 for zero, e + CODE_BIAS for a finite nonzero x,
+// CODE_BAD for Inf / NaN.  A line's largest code is its maximum's code
+// (frexp exponents order like magnitudes), taken with integer atomicMax.
+constexpr int NO_EXP = -(1 << 20);
+constexpr int CODE_BIAS = 2048;
+constexpr unsigned long long CODE_BAD = 0x7fffffffull;
+constexpr int CLASS_FINITE_NONZERO = 0x198;   // -normal, -denormal, +denormal, +normal
+constexpr int CLASS_INF_NAN = 0x207;          // signalling / quiet NaN, -Inf, +Inf
+__device__ inline void elem_exp(double x, int &e, unsigned &code) {
+    const int fe = __builtin_amdgcn_frexp_exp(x);
+    const bool ok = __builtin_amdgcn_class(x, CLASS_FINITE_NONZERO);
+    const bool bad = __builtin_amdgcn_class(x, CLASS_INF_NAN);
+    e = ok ? fe : NO_EXP;
+    code = ok ? (unsigned)(fe + CODE_BIAS) : bad ? (unsigned)CODE_BAD : 0u;
+}
+
+// Group term of mgx::modular's exponent rule, 0 for an element without an
+// exponent (the group maximum g is at least e).
+__device__ inline unsigned group_term32(int g, int e) {
+    const int d = min(g - e, mgx::modular::HIST_W);
+    return e != NO_EXP ? 1u << (2 * (mgx::modular::HIST_W - d)) : 0u;
+}
+
+// Row scan of A (m x k, lda): the row-maximum codes and, for the modular form
+// (SUMS), the exponent-group sums N of the per-line scaling rule
+// (mgx::modular, groups of 16 consecutive k).  grid.x covers k in segments of
+// 256*16, grid.y strides over rows.  For a fixed i the 64 lanes of a wave
+// hold 64 consecutive k, so a group is one 16-lane DPP row, whose maximum
+// takes four DPP steps (quad swaps, then the half-row and row mirrors; every
+// lane of the wave is active).  Codes and sums
+// reach memory through integer atomicMax / atomicAdd, so the result does not
+// depend on the order of the workgroups.
+template <bool SUMS>
+__global__ __launch_bounds__(256) void row_scan_kernel(const double *__restrict__ A, int64_t m,
+                                                       int64_t k, int64_t lda,
+                                                       unsigned long long *rowmax,
+                                                       unsigned long long *rowsum) {
+    __shared__ unsigned long long sh[4], ss[4];
     const int64_t kbase = (int64_t)blockIdx.x * (256 * 16);
     for (int64_t r = blockIdx.y; r < m; r += gridDim.y) {
-        unsigned long long mx = 0;
-        const double *row = A + r * lda;
-#pragma unroll 4
-        for (int i = 0; i < 16; ++i) {
-            const int64_t kk = kbase + i * 256 + threadIdx.x;
-            if (kk < k) mx = max(mx, abs_bits(__builtin_nontemporal_load(&row[kk])));
+        const double *row = A + r * lda + kbase + threadIdx.x;
+        double v[16];
+        if (kbase + 256 * 16 <= k) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) v[i] = __builtin_nontemporal_load(&row[i * 256]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                v[i] = kbase + i * 256 + threadIdx.x < k ? __builtin_nontemporal_load(&row[i * 256])
+                                                         : 0.0;
         }
-        for (int off = 32; off > 0; off >>= 1) mx = max(mx, (unsigned long long)__shfl_xor(mx, off));
-        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+        unsigned mx = 0, ns = 0;   // ns <= 16 * 4^12 = 2^28
+        int e[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            unsigned code;
+            elem_exp(v[i], e[i], code);
+            mx = max(mx, code);
+        }
+        if constexpr (SUMS) {
+            // the 16 group maxima stage by stage, so consecutive DPP steps
+            // work on different values
+            int g[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                g[i] = max(e[i], __builtin_amdgcn_update_dpp(e[i], e[i], 0xB1, 0xF, 0xF, true));
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                g[i] = max(g[i], __builtin_amdgcn_update_dpp(g[i], g[i], 0x4E, 0xF, 0xF, true));
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                g[i] = max(g[i], __builtin_amdgcn_update_dpp(g[i], g[i], 0x141, 0xF, 0xF, true));
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                g[i] = max(g[i], __builtin_amdgcn_update_dpp(g[i], g[i], 0x140, 0xF, 0xF, true));
+#pragma unroll
+            for (int i = 0; i < 16; ++i) ns += group_term32(g[i], e[i]);
+        }
+        unsigned long long mx64 = mx, ns64 = ns;
+        for (int off = 32; off > 0; off >>= 1) {
+            mx64 = max(mx64, (unsigned long long)__shfl_xor(mx64, off));
+            if constexpr (SUMS) ns64 += (unsigned long long)__shfl_xor(ns64, off);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            sh[threadIdx.x >> 6] = mx64;
+            ss[threadIdx.x >> 6] = ns64;
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
-            mx = max(max(sh[0], sh[1]), max(sh[2], sh[3]));
-            if (mx) atomicMax(rowmax + r, mx);
+            mx64 = max(max(sh[0], sh[1]), max(sh[2], sh[3]));
+            if (mx64) atomicMax(rowmax + r, mx64);
+            if constexpr (SUMS) {
+                ns64 = ss[0] + ss[1] + ss[2] + ss[3];
+                if (ns64) atomicAdd(rowsum + r, ns64);
+            }
         }
         __syncthreads();
     }
 }
 
-// Column maxima of B (k x n, ldb): one lane per column, grid.y splits k in
-// segments of 256 rows.
-__global__ __launch_bounds__(256) void col_absmax_kernel(const double *__restrict__ B, int64_t k,
-                                                         int64_t n, int64_t ldb,
-                                                         unsigned long long *colmax) {
+// Column scan of B (k x n, ldb): one lane per column, grid.y splits k in
+// segments of 256 rows, read 16 rows (one group) at a time.
+template <bool SUMS>
+__global__ __launch_bounds__(256) void col_scan_kernel(const double *__restrict__ B, int64_t k,
+                                                       int64_t n, int64_t ldb,
+                                                       unsigned long long *colmax,
+                                                       unsigned long long *colsum) {
     const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (c >= n) return;
     const int64_t k0 = (int64_t)blockIdx.y * 256;
     const int64_t k1 = min(k, k0 + 256);
-    unsigned long long mx = 0;
-    // read-once stream: non-temporal loads
-    for (int64_t kk = k0; kk < k1; ++kk)
-        mx = max(mx, abs_bits(__builtin_nontemporal_load(&B[kk * ldb + c])));
-    if (mx) atomicMax(colmax + c, mx);
+    unsigned mx = 0;
+    unsigned long long ns = 0;
+    for (int64_t kq = k0; kq < k1; kq += 16) {
+        double v[16];
+        const double *col = B + kq * ldb + c;
+        // read-once stream: non-temporal loads
+        if (kq + 16 <= k1) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) v[i] = __builtin_nontemporal_load(&col[i * ldb]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                v[i] = kq + i < k1 ? __builtin_nontemporal_load(&col[i * ldb]) : 0.0;
+        }
+        int e[16], g = NO_EXP;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            unsigned code;
+            elem_exp(v[i], e[i], code);
+            mx = max(mx, code);
+            g = max(g, e[i]);
+        }
+        if constexpr (SUMS) {
+            unsigned gs = 0;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) gs += group_term32(g, e[i]);
+            ns += gs;
+        }
+    }
+    if (mx) atomicMax(colmax + c, (unsigned long long)mx);
+    if constexpr (SUMS)
+        if (ns) atomicAdd(colsum + c, ns);
 }
 
-__device__ inline int exponent_of_bits(unsigned long long bits) {
-    if (bits == 0) return 0;
-    if (bits >= 0x7ff0000000000000ull) return EXP_BAD;
-    int e;
-    frexp(__longlong_as_double((long long)bits), &e);
-    return e;
+// Line exponent from a line-maximum code (0 for an all-zero line).
+__device__ inline int exponent_of_code(unsigned long long code) {
+    if (code == 0) return 0;
+    if (code >= CODE_BAD) return EXP_BAD;
+    return (int)code - CODE_BIAS;
 }
 
-__global__ void exponents_kernel(const unsigned long long *maxbits, int32_t *e, int64_t n) {
+// Line exponents e from the maximum codes; for the modular form (MOD) also
+// the line's scale shift e - beta, beta the line's scaling exponent from its
+// group sum (mgx::modular::line_beta); EXP_BAD for a non-finite line.  The
+// scaled line is X 2^-shift and the product's exponent is the sum of the two
+// lines' shifts.
+template <bool MOD>
+__global__ void line_scales_kernel(const unsigned long long *maxcode,
+                                   const unsigned long long *sums, int32_t *e, int32_t *shift,
+                                   int64_t n, mgx::modular::BetaThresholds T) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        e[i] = exponent_of_bits(maxbits[i]);
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ei = exponent_of_code(maxcode[i]);
+        e[i] = ei;
+        if constexpr (MOD)
+            shift[i] = ei == EXP_BAD ? EXP_BAD : ei - mgx::modular::line_beta(T, sums[i]);
+    }
 }
 
 // ---- slice extraction -----------------------------------------------------
@@ -252,7 +368,7 @@ __device__ inline void emit_residues16(const double *x, int n, int8_t *dst, int6
 constexpr int ROW_ALIGN = 256;
 
 // Per-element operand value: scheme I scales into (-1,1), scheme II to an
-// integer of at most beta bits.
+// integer of at most beta bits (beta = the line's scaling exponent).
 template <bool MOD>
 __device__ inline double scaled_operand(double a, int e, int beta) {
     return MOD ? rint(ldexp(a, beta - e)) : ldexp(a, -e);
@@ -274,8 +390,9 @@ template <bool MOD, bool COLS, bool BF = false>
 __global__ __launch_bounds__(256) void operand_tile_kernel(const double *__restrict__ X,
                                                            int64_t lines, int64_t k, int64_t ld,
                                                            const int32_t *__restrict__ ex,
-                                                           int planes, int beta, int64_t Lp,
-                                                           int64_t KTC, int8_t *__restrict__ Xop) {
+                                                           const int32_t *__restrict__ lshift,
+                                                           int planes, int64_t Lp, int64_t KTC,
+                                                           int8_t *__restrict__ Xop) {
     constexpr int STR = 66;   // doubles per padded LDS row
     __shared__ double tile[64 * STR];
     // Consecutive workgroups take consecutive 512-byte segments of the same
@@ -322,6 +439,7 @@ __global__ __launch_bounds__(256) void operand_tile_kernel(const double *__restr
     const int kq = lane >> 4;
     const int64_t gl = l0 + line;
     const int e = gl < lines ? ex[gl] : EXP_BAD;
+    const int beta = MOD && gl < lines && e != EXP_BAD ? e - lshift[gl] : 0;
     double x[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
@@ -338,7 +456,8 @@ __global__ __launch_bounds__(256) void operand_tile_kernel(const double *__restr
     }
     int8_t *dst = Xop + (kt * Lp + gl) * KT + kq * 16;
     if (MOD) {
-        // |A'| <= 2^beta: beta <= 51 takes the shorter residue sequence
+        // |X'| <= 2^beta: lines with beta <= 51 take the shorter residue
+        // sequence (beta is per line, so a wave may take both)
         if (beta <= 51)
             emit_residues16<true>(x, planes, dst, KTC * Lp * KT);
         else
@@ -873,11 +992,11 @@ __device__ __forceinline__ double crt_value(const uint32_t (&res)[MAX_MODULI], i
 
 template <int TB, int WGM, int WGN, int NM>
 __global__ __launch_bounds__(256) void reconstruct_kernel(const uint8_t *__restrict__ R,
-                                                          const int32_t *__restrict__ rexp,
-                                                          const int32_t *__restrict__ cexp,
+                                                          const int32_t *__restrict__ rshift,
+                                                          const int32_t *__restrict__ cshift,
                                                           double *__restrict__ C, int64_t M,
                                                           int64_t N, int64_t ldc, int nmod,
-                                                          int beta, int accumulate) {
+                                                          int accumulate) {
     using S = TileShape<TB, TB, WGM, WGN>;
     const int nm = NM > 0 ? NM : nmod;
     const int tiles_m = (int)((M + TB - 1) / TB);
@@ -893,7 +1012,6 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(const uint8_t *__restr
     const int64_t n0 = (tile % tiles_n) * TB;
     const int64_t row0 = m0 + (wave / WGN) * S::WTM + 4 * (lane >> 4);
     const int64_t col0 = n0 + (wave % WGN) * S::WTN + (lane & 15);
-    const int shift = -2 * beta;
     {
         v4i runs[MAX_MODULI];
 #pragma unroll
@@ -905,7 +1023,7 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(const uint8_t *__restr
         for (int w = 0; w < 4; ++w) {
             const int f = 4 * q + w;
             const int64_t col = col0 + 16 * (f % S::FJ);
-            const int ec = col < N ? cexp[col] : 0;
+            const int ec = col < N ? cshift[col] : 0;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t row = row0 + 16 * (f / S::FJ) + r;
@@ -914,11 +1032,10 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(const uint8_t *__restr
 #pragma unroll
                     for (int l = 0; l < MAX_MODULI; ++l)
                         res[l] = l < nm ? ((uint32_t)runs[l][w] >> (8 * r)) & 0xffu : 0u;
-                    const int er = rexp[row];
+                    const int er = rshift[row];
                     const double x = (er == EXP_BAD || ec == EXP_BAD)
                                          ? __longlong_as_double(0x7ff8000000000000ll)
-                                         : ldexp(crt_value<NM>(res, nm),
-                                                 er + ec + shift);
+                                         : ldexp(crt_value<NM>(res, nm), er + ec);
                     double *c = C + row * ldc + col;
                     *c = accumulate ? *c + x : x;
                 }
@@ -952,7 +1069,8 @@ bool form_planes_ok(int form, int planes) {
 
 struct SplitLayout {
     int64_t Kp, KTC, Mp, Np;
-    size_t a_off, b_off, r_off, rmax_off, cmax_off, rexp_off, cexp_off, total;
+    size_t a_off, b_off, r_off, rmax_off, cmax_off, rsum_off, csum_off, rexp_off, cexp_off;
+    size_t rshift_off, cshift_off, total;
 };
 
 SplitLayout split_layout(int form, int planes, int64_t m, int64_t n, int64_t k) {
@@ -973,10 +1091,16 @@ SplitLayout split_layout(int form, int planes, int64_t m, int64_t n, int64_t k) 
     L.b_off = take((size_t)planes * L.Np * L.Kp * esize);
     // product residues: every block tile of either tile size fits in Mp x Np
     L.r_off = take(form == MGX_FORM_INT8_MOD ? (size_t)planes * L.Mp * L.Np : 0);
-    L.rmax_off = take((size_t)(m + n) * 8);
+    // row / column maxima, then the modular form's group sums: one block of
+    // 2 (m + n) words cleared by one kernel
+    L.rmax_off = take((size_t)(m + n) * 16);
     L.cmax_off = L.rmax_off + (size_t)m * 8;
+    L.rsum_off = L.cmax_off + (size_t)n * 8;
+    L.csum_off = L.rsum_off + (size_t)m * 8;
     L.rexp_off = take((size_t)m * 4);
     L.cexp_off = take((size_t)n * 4);
+    L.rshift_off = take((size_t)m * 4);
+    L.cshift_off = take((size_t)n * 4);
     L.total = off;
     return L;
 }
@@ -1048,8 +1172,11 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     }
     MGX_ARG_CHECK(A != nullptr && B != nullptr, "A or B is NULL");
     MGX_ARG_CHECK(lda >= k && ldb >= n, "lda < k or ldb < n");
-    const int beta = mod ? mgx::modular::beta_for(slices, k) : 0;
-    if (mod && beta < 0)
+    // per-line scaling exponents of the modular form: every line gets at
+    // least beta_for(slices, k) >= 0
+    mgx::modular::BetaThresholds thresholds = {};
+    if (mod && (mgx::modular::beta_for(slices, k) < 0 ||
+                !mgx::modular::beta_thresholds(slices, k, &thresholds)))
         return mgx::set_error(MGX_ERR_UNSUPPORTED, "too few moduli for this k");
     const SplitLayout L = split_layout(form, slices, m, n, k);
     const int64_t tiles = ((m + TBm - 1) / TBm) * ((n + TBn - 1) / TBn);
@@ -1091,8 +1218,12 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     uint8_t *Rres = (uint8_t *)(ws + L.r_off);
     unsigned long long *rmax = (unsigned long long *)(ws + L.rmax_off);
     unsigned long long *cmax = (unsigned long long *)(ws + L.cmax_off);
+    unsigned long long *rsum = (unsigned long long *)(ws + L.rsum_off);
+    unsigned long long *csum = (unsigned long long *)(ws + L.csum_off);
     int32_t *rexp = (int32_t *)(ws + L.rexp_off);
     int32_t *cexp = (int32_t *)(ws + L.cexp_off);
+    int32_t *rshift = (int32_t *)(ws + L.rshift_off);
+    int32_t *cshift = (int32_t *)(ws + L.cshift_off);
     int rc;
     auto mark = [&](int i) -> int {
         if (events && events[i]) MGX_HIP_TRY(hipEventRecord((hipEvent_t)events[i], st));
@@ -1100,25 +1231,43 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     };
 
     if ((rc = mark(0)) != MGX_OK) return rc;
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1d(m + n, 256)), dim3(256), 0, st, rmax, m + n);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1d(2 * (m + n), 256)), dim3(256), 0, st, rmax,
+                       2 * (m + n));
     if ((rc = mgx::launch_status("zero_u64_kernel")) != MGX_OK) return rc;
     {
         const unsigned gx = (unsigned)((k + 4095) / 4096);
         const unsigned gy = (unsigned)std::min<int64_t>(m, 65535);
-        hipLaunchKernelGGL(row_absmax_kernel, dim3(gx, gy), dim3(256), 0, st, A, m, k, lda, rmax);
-        if ((rc = mgx::launch_status("row_absmax_kernel")) != MGX_OK) return rc;
+        if (mod)
+            hipLaunchKernelGGL(row_scan_kernel<true>, dim3(gx, gy), dim3(256), 0, st, A, m, k, lda,
+                               rmax, rsum);
+        else
+            hipLaunchKernelGGL(row_scan_kernel<false>, dim3(gx, gy), dim3(256), 0, st, A, m, k,
+                               lda, rmax, rsum);
+        if ((rc = mgx::launch_status("row_scan_kernel")) != MGX_OK) return rc;
     }
     {
         const unsigned gx = (unsigned)((n + 255) / 256);
         const int64_t gy = (k + 255) / 256;
-        hipLaunchKernelGGL(col_absmax_kernel, dim3(gx, (unsigned)gy), dim3(256), 0, st, B, k, n,
-                           ldb, cmax);
-        if ((rc = mgx::launch_status("col_absmax_kernel")) != MGX_OK) return rc;
+        if (mod)
+            hipLaunchKernelGGL(col_scan_kernel<true>, dim3(gx, (unsigned)gy), dim3(256), 0, st, B,
+                               k, n, ldb, cmax, csum);
+        else
+            hipLaunchKernelGGL(col_scan_kernel<false>, dim3(gx, (unsigned)gy), dim3(256), 0, st, B,
+                               k, n, ldb, cmax, csum);
+        if ((rc = mgx::launch_status("col_scan_kernel")) != MGX_OK) return rc;
     }
-    hipLaunchKernelGGL(exponents_kernel, dim3(grid1d(m, 256)), dim3(256), 0, st, rmax, rexp, m);
-    if ((rc = mgx::launch_status("exponents_kernel")) != MGX_OK) return rc;
-    hipLaunchKernelGGL(exponents_kernel, dim3(grid1d(n, 256)), dim3(256), 0, st, cmax, cexp, n);
-    if ((rc = mgx::launch_status("exponents_kernel")) != MGX_OK) return rc;
+    if (mod) {
+        hipLaunchKernelGGL(line_scales_kernel<true>, dim3(grid1d(m, 256)), dim3(256), 0, st, rmax,
+                           rsum, rexp, rshift, m, thresholds);
+        hipLaunchKernelGGL(line_scales_kernel<true>, dim3(grid1d(n, 256)), dim3(256), 0, st, cmax,
+                           csum, cexp, cshift, n, thresholds);
+    } else {
+        hipLaunchKernelGGL(line_scales_kernel<false>, dim3(grid1d(m, 256)), dim3(256), 0, st, rmax,
+                           rsum, rexp, rshift, m, thresholds);
+        hipLaunchKernelGGL(line_scales_kernel<false>, dim3(grid1d(n, 256)), dim3(256), 0, st, cmax,
+                           csum, cexp, cshift, n, thresholds);
+    }
+    if ((rc = mgx::launch_status("line_scales_kernel")) != MGX_OK) return rc;
     if ((rc = mark(1)) != MGX_OK) return rc;
 
     {
@@ -1127,13 +1276,13 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
         const dim3 grid((unsigned)(L.Kp / KT), (unsigned)gy);
         if (bf)
             hipLaunchKernelGGL((operand_tile_kernel<false, false, true>), grid, dim3(256), 0, st,
-                               A, m, k, lda, rexp, slices, 0, L.Mp, L.KTC, Aop);
+                               A, m, k, lda, rexp, rshift, slices, L.Mp, L.KTC, Aop);
         else if (mod)
             hipLaunchKernelGGL((operand_tile_kernel<true, false>), grid, dim3(256), 0, st, A, m, k,
-                               lda, rexp, slices, beta, L.Mp, L.KTC, Aop);
+                               lda, rexp, rshift, slices, L.Mp, L.KTC, Aop);
         else
             hipLaunchKernelGGL((operand_tile_kernel<false, false>), grid, dim3(256), 0, st, A, m, k,
-                               lda, rexp, slices, 0, L.Mp, L.KTC, Aop);
+                               lda, rexp, rshift, slices, L.Mp, L.KTC, Aop);
         if ((rc = mgx::launch_status("operand_tile_kernel (A)")) != MGX_OK) return rc;
     }
     {
@@ -1141,13 +1290,13 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
         const dim3 grid((unsigned)(L.Np / 64), (unsigned)(L.Kp / KT));
         if (bf)
             hipLaunchKernelGGL((operand_tile_kernel<false, true, true>), grid, dim3(256), 0, st,
-                               B, n, k, ldb, cexp, slices, 0, L.Np, L.KTC, Bop);
+                               B, n, k, ldb, cexp, cshift, slices, L.Np, L.KTC, Bop);
         else if (mod)
             hipLaunchKernelGGL((operand_tile_kernel<true, true>), grid, dim3(256), 0, st, B, n, k,
-                               ldb, cexp, slices, beta, L.Np, L.KTC, Bop);
+                               ldb, cexp, cshift, slices, L.Np, L.KTC, Bop);
         else
             hipLaunchKernelGGL((operand_tile_kernel<false, true>), grid, dim3(256), 0, st, B, n, k,
-                               ldb, cexp, slices, 0, L.Np, L.KTC, Bop);
+                               ldb, cexp, cshift, slices, L.Np, L.KTC, Bop);
         if ((rc = mgx::launch_status("operand_tile_kernel (B)")) != MGX_OK) return rc;
     }
     if ((rc = mark(2)) != MGX_OK) return rc;
@@ -1192,8 +1341,8 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
         auto launch = [&](auto tb, auto nm) {
             constexpr int TBv = decltype(tb)::value, NMv = decltype(nm)::value;
             hipLaunchKernelGGL((reconstruct_kernel<TBv, 2, TBv == 256 ? 4 : 2, NMv>),
-                               dim3((unsigned)blocks), dim3(256), 0, st, Rres, rexp, cexp, C, m,
-                               n, ldc, slices, beta, accumulate ? 1 : 0);
+                               dim3((unsigned)blocks), dim3(256), 0, st, Rres, rshift, cshift, C,
+                               m, n, ldc, slices, accumulate ? 1 : 0);
         };
         using L = std::integral_constant<int, 256>;
         using Sm = std::integral_constant<int, 128>;

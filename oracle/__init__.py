@@ -46,6 +46,7 @@ _SIG = {
     "orc_crt_modulus": (_int, [_int]),
     "orc_crt_log2_product": (_dbl, [_int]),
     "orc_crt_beta": (_int, [_int, _i64]),
+    "orc_crt_line_betas": (_int, [_int, _vp, _i64, _i64, _i64, _i64, _vp]),
     "orc_crt_gemm": (_int, [_int, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _int, _int]),
     "orc_mlp_param_count": (_i64, [_int]),
     "orc_mlp_forward": (None, [_vp, _int, _vp, _i64, _vp]),
@@ -220,8 +221,24 @@ def crt_log2_product(nmod: int) -> float:
 
 
 def crt_beta(nmod: int, k: int) -> int:
-    """Scaling exponent of the modular form: largest beta <= 53 with 2 k 4^beta < P_n."""
+    """Exponent every line gets at least in the modular form (the exponent of
+    a line whose elements all sit in its top binade); -1 when n moduli cannot
+    hold depth k."""
     return int(lib().orc_crt_beta(nmod, k))
+
+
+def crt_line_betas(X: np.ndarray, nmod: int, axis: int = 1) -> np.ndarray:
+    """Per-line scaling exponents of the modular form: rows of X (axis=1, the
+    rows of A) or columns of X (axis=0, the columns of B)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    rows, cols = X.shape
+    lines, k = (rows, cols) if axis == 1 else (cols, rows)
+    lstride, kstride = (cols, 1) if axis == 1 else (1, cols)
+    out = np.empty(lines, dtype=np.int32)
+    rc = lib().orc_crt_line_betas(nmod, _p(X), lines, k, lstride, kstride, _p(out))
+    if rc != 0:
+        raise ValueError(f"{nmod} moduli cannot hold depth {k}")
+    return out
 
 
 def crt_gemm(A: np.ndarray, B: np.ndarray, nmod: int, C: np.ndarray | None = None,

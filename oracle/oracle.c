@@ -417,15 +417,24 @@ ORC_API int orc_split_gemm_int8(int s, const double *A, const double *B, double 
 /* ---- INT8 modular form (Ozaki scheme II) ---------------------------------- */
 /* Published algorithm: K. Ozaki, Y. Uchino, T. Imamura, "Ozaki Scheme II: A
  * GEMM-oriented emulation of floating-point matrix multiplication using an
- * integer modular technique" (2025).  A row i is scaled to integers
- * A'_ik = rint(A_ik 2^(beta-e_i)) (|A'| <= 2^beta), B likewise per column with
- * f_j; the integer product C' = A'B' is formed exactly (libmgx forms it from
- * INT8 products modulo n pairwise coprime moduli and Chinese remaindering;
- * this model forms it directly in 128-bit integers, which is the same integer
- * whenever |C'| < P/2, P = m_0 ... m_{n-1}, which beta guarantees), and
- *   C_ij = ldexp((double)C'_ij, e_i + f_j - 2 beta)     (one rounding, RN-even)
- * (or C_in + that for the accumulate form).  beta is the largest integer in
- * [0, 53] with 2 k 4^beta < P. */
+ * integer modular technique" (2025).  Row i is scaled to integers
+ * A'_ik = rint(A_ik 2^(beta_i-e_i)) (|A'| <= 2^beta_i), column j of B likewise
+ * with f_j and delta_j; the integer product C' = A'B' is formed exactly
+ * (libmgx forms it from INT8 products modulo n pairwise coprime moduli and
+ * Chinese remaindering; this model forms it directly in 128-bit integers,
+ * which is the same integer whenever |C'| < P/2, P = m_0 ... m_{n-1}, which
+ * the per-line exponents guarantee), and
+ *   C_ij = ldexp((double)C'_ij, e_i - beta_i + f_j - delta_j)   (one rounding)
+ * (or C_in + that for the accumulate form).
+ *
+ * Per-line exponent (this build's own rule; Cauchy-Schwarz on an exponent
+ * histogram): for a line x_1..x_k (a row of A or a column of B) with e(x) the
+ * frexp exponent and groups of 16 consecutive k,
+ *   N = sum_groups sum_{x != 0 in group} 4^(12 - min(g - e(x), 12)),
+ *   g = the group's largest e(x);
+ * R = floor(sqrt(P/2)) - ceil(ceil(sqrt(k))/2) - 1; beta is the largest
+ * integer in [0, 53] with 4^beta N <= R^2 4^12.  Then sum x^2 < 4^e N/4^12,
+ * ||X'||_2 < R + sqrt(k)/2 and |C'_ij| <= ||A'_i|| ||B'_j|| < P/2. */
 #define ORC_MAX_MODULI 16
 static const uint32_t orc_moduli[ORC_MAX_MODULI] = {256, 255, 253, 251, 247, 241, 239, 233,
                                                      229, 227, 223, 217, 211, 199, 197, 193};
@@ -445,17 +454,88 @@ ORC_API double orc_crt_log2_product(int nmod) {
     return s;
 }
 
-ORC_API int orc_crt_beta(int nmod, int64_t k) {
-    if (nmod < 2 || nmod > ORC_MAX_MODULI || k < 1) return -1;
+#define ORC_HIST_W 12
+#define ORC_GROUP_K 16
+
+static unsigned __int128 orc_isqrt(unsigned __int128 x) {
+    unsigned __int128 r = (unsigned __int128)sqrtl((long double)x);
+    while (r > 0 && r * r > x) --r;
+    while ((r + 1) * (r + 1) <= x) ++r;
+    return r;
+}
+
+/* R^2 of the per-line rule, 0 when R < 1. */
+static unsigned __int128 orc_crt_r2(int nmod, int64_t k) {
     const unsigned __int128 P = orc_crt_product(nmod);
-    unsigned __int128 lhs = (unsigned __int128)2 * (uint64_t)k; /* 2 k 4^beta */
-    if (lhs >= P) return -1;
-    int beta = 0;
-    while (beta < 53 && lhs <= (P - 1) / 4 && lhs * 4 < P) {
-        lhs *= 4;
-        ++beta;
+    const unsigned __int128 s = orc_isqrt(P / 2);
+    unsigned __int128 h = orc_isqrt((unsigned __int128)k);
+    if (h * h < (unsigned __int128)k) h += 1;
+    const unsigned __int128 cut = (h + 1) / 2 + 1;
+    if (s <= cut) return 0;
+    return (s - cut) * (s - cut);
+}
+
+/* Largest beta in [0, 53] with 4^beta N <= R2 4^12, or -1. */
+static int orc_crt_beta_of(unsigned __int128 R2, uint64_t N) {
+    for (int b = 53; b >= 0; --b) {
+        /* 4^b N <= R2 4^12  <=>  N 4^(b-12) <= R2 (b >= 12), N <= R2 4^(12-b) */
+        if (b >= ORC_HIST_W) {
+            /* N 2^sh <= R2  <=>  N <= floor(R2 / 2^sh) */
+            if ((unsigned __int128)N <= (R2 >> (2 * (b - ORC_HIST_W)))) return b;
+        } else {
+            /* N <= R2 2^sh  <=>  ceil(N / 2^sh) <= R2 */
+            const int sh = 2 * (ORC_HIST_W - b);
+            if ((((unsigned __int128)N + ((unsigned __int128)1 << sh) - 1) >> sh) <= R2) return b;
+        }
     }
-    return beta;
+    return -1;
+}
+
+/* Exponent of the densest line (every element in the top binade,
+ * N = k 4^12): the exponent every line gets at least. */
+ORC_API int orc_crt_beta(int nmod, int64_t k) {
+    if (nmod < 2 || nmod > ORC_MAX_MODULI || k < 1 || k > ((int64_t)1 << 36)) return -1;
+    const unsigned __int128 R2 = orc_crt_r2(nmod, k);
+    if (R2 == 0) return -1;
+    return orc_crt_beta_of(R2, (uint64_t)k << (2 * ORC_HIST_W));
+}
+
+/* N of one line: x(i) = X[i * kstride], i < k. */
+static uint64_t orc_crt_line_sum(const double *X, int64_t k, int64_t kstride) {
+    uint64_t N = 0;
+    for (int64_t g0 = 0; g0 < k; g0 += ORC_GROUP_K) {
+        const int64_t g1 = g0 + ORC_GROUP_K < k ? g0 + ORC_GROUP_K : k;
+        int top = INT32_MIN;
+        for (int64_t i = g0; i < g1; ++i) {
+            const double x = X[i * kstride];
+            if (x != 0.0 && isfinite(x)) {
+                int e;
+                frexp(x, &e);
+                if (e > top) top = e;
+            }
+        }
+        for (int64_t i = g0; i < g1; ++i) {
+            const double x = X[i * kstride];
+            if (x != 0.0 && isfinite(x)) {
+                int e;
+                frexp(x, &e);
+                const int d = top - e < ORC_HIST_W ? top - e : ORC_HIST_W;
+                N += (uint64_t)1 << (2 * (ORC_HIST_W - d));
+            }
+        }
+    }
+    return N;
+}
+
+/* Per-line exponents of `lines` lines: line l is X[l * lstride + i * kstride],
+ * i < k.  Returns 0, or -1 when the moduli cannot hold depth k. */
+ORC_API int orc_crt_line_betas(int nmod, const double *X, int64_t lines, int64_t k,
+                               int64_t lstride, int64_t kstride, int32_t *beta) {
+    if (orc_crt_beta(nmod, k) < 0) return -1;
+    const unsigned __int128 R2 = orc_crt_r2(nmod, k);
+    for (int64_t l = 0; l < lines; ++l)
+        beta[l] = orc_crt_beta_of(R2, orc_crt_line_sum(X + l * lstride, k, kstride));
+    return 0;
 }
 
 ORC_API int orc_crt_gemm(int nmod, const double *A, const double *B, double *C, int64_t m,
@@ -469,27 +549,30 @@ ORC_API int orc_crt_gemm(int nmod, const double *A, const double *B, double *C, 
                 for (int64_t j = 0; j < n; ++j) C[i * ldc + j] = 0.0;
         return 0;
     }
-    const int beta = orc_crt_beta(nmod, k);
-    if (beta < 0) return -1;
+    if (orc_crt_beta(nmod, k) < 0) return -1;
     int32_t *rexp = (int32_t *)malloc(sizeof(int32_t) * m);
     int32_t *cexp = (int32_t *)malloc(sizeof(int32_t) * n);
+    int32_t *rbeta = (int32_t *)malloc(sizeof(int32_t) * m);
+    int32_t *cbeta = (int32_t *)malloc(sizeof(int32_t) * n);
     int64_t *Ai = (int64_t *)malloc(sizeof(int64_t) * m * k);
     int64_t *Bi = (int64_t *)malloc(sizeof(int64_t) * n * k); /* transposed: [j][kk] */
-    if (!rexp || !cexp || !Ai || !Bi) {
-        free(rexp); free(cexp); free(Ai); free(Bi);
+    if (!rexp || !cexp || !rbeta || !cbeta || !Ai || !Bi) {
+        free(rexp); free(cexp); free(rbeta); free(cbeta); free(Ai); free(Bi);
         return -2;
     }
     orc_split_exponents(A, B, m, n, k, lda, ldb, rexp, cexp);
+    orc_crt_line_betas(nmod, A, m, k, lda, 1, rbeta);
+    orc_crt_line_betas(nmod, B, n, k, 1, ldb, cbeta);
     for (int64_t i = 0; i < m; ++i)
         for (int64_t kk = 0; kk < k; ++kk)
             Ai[i * k + kk] = rexp[i] == ORC_EXP_BAD
                                  ? 0
-                                 : (int64_t)rint(ldexp(A[i * lda + kk], beta - rexp[i]));
+                                 : (int64_t)rint(ldexp(A[i * lda + kk], rbeta[i] - rexp[i]));
     for (int64_t j = 0; j < n; ++j)
         for (int64_t kk = 0; kk < k; ++kk)
             Bi[j * k + kk] = cexp[j] == ORC_EXP_BAD
                                  ? 0
-                                 : (int64_t)rint(ldexp(B[kk * ldb + j], beta - cexp[j]));
+                                 : (int64_t)rint(ldexp(B[kk * ldb + j], cbeta[j] - cexp[j]));
 #pragma omp parallel for schedule(dynamic, 1) collapse(2)
     for (int64_t i = 0; i < m; ++i) {
         for (int64_t j = 0; j < n; ++j) {
@@ -498,11 +581,12 @@ ORC_API int orc_crt_gemm(int nmod, const double *A, const double *B, double *C, 
             __int128 acc = 0;
             for (int64_t kk = 0; kk < k; ++kk) acc += (__int128)ar[kk] * br[kk];
             const int bad = rexp[i] == ORC_EXP_BAD || cexp[j] == ORC_EXP_BAD;
-            const double x = bad ? NAN : ldexp((double)acc, rexp[i] + cexp[j] - 2 * beta);
+            const double x =
+                bad ? NAN : ldexp((double)acc, rexp[i] - rbeta[i] + cexp[j] - cbeta[j]);
             C[i * ldc + j] = accumulate ? C[i * ldc + j] + x : x;
         }
     }
-    free(rexp); free(cexp); free(Ai); free(Bi);
+    free(rexp); free(cexp); free(rbeta); free(cbeta); free(Ai); free(Bi);
     return 0;
 }
 

@@ -135,6 +135,44 @@ int main() {
             }
         }
     }
-    printf("reconstructions %ld residues %ld bad %ld\n", checked, residues, bad);
+    // per-line exponents: the threshold search against the definition
+    // (largest b in [0, 53] with 4^b N <= R^2 4^W) evaluated by exact division
+    long lines = 0;
+    const int64_t depths[] = {1, 7, 64, 1000, 4096, 16384, 32768, 131072};
+    for (int n = MIN_MODULI; n <= MAX_MODULI; ++n) {
+        for (int64_t k : depths) {
+            BetaThresholds T;
+            const u128 R2 = radius2(n, k);
+            if (!beta_thresholds(n, k, &T)) {
+                if (R2 != 0) { printf("thresholds refused n=%d k=%lld\n", n, (long long)k); ++bad; }
+                continue;
+            }
+            const unsigned long long nmax = (unsigned long long)k << (2 * HIST_W);
+            for (int it = 0; it < 4000; ++it) {
+                unsigned long long N = it == 0 ? nmax : it == 1 ? 0 : g() % (nmax + 1);
+                int want = -1;
+                for (int b = 0; b <= BETA_MAX; ++b) {
+                    const bool ok = b >= HIST_W ? (u128)N <= (R2 >> (2 * (b - HIST_W)))
+                                                : (((u128)N + ((u128)1 << (2 * (HIST_W - b))) - 1)
+                                                   >> (2 * (HIST_W - b))) <= R2;
+                    if (ok) want = b;
+                }
+                const int got = line_beta(T, N);
+                ++lines;
+                if (got != want) {
+                    if (bad < 20) printf("line_beta n=%d k=%lld N=%llu: %d want %d\n", n,
+                                         (long long)k, N, got, want);
+                    ++bad;
+                }
+                if (it == 0 && got != beta_for(n, k)) {
+                    printf("beta_for n=%d k=%lld: %d vs dense line %d\n", n, (long long)k,
+                           beta_for(n, k), got);
+                    ++bad;
+                }
+            }
+        }
+    }
+    printf("reconstructions %ld residues %ld line exponents %ld bad %ld\n", checked, residues,
+           lines, bad);
     return bad != 0;
 }

@@ -53,6 +53,51 @@ def test_bit_exact_vs_model_stress(mgx, orc, nmod):
     assert _same(_mod(mgx, A, B, nmod), orc.crt_gemm(A, B, nmod))
 
 
+@pytest.mark.parametrize("nmod", [9, 13, 15])
+def test_per_line_exponents_subnormals_and_histogram_edges(mgx, orc, nmod):
+    # Lines whose exponent-group sums exercise every branch of the per-line
+    # rule: subnormal and mixed normal / subnormal lines, elements exactly 11,
+    # 12 and 13 binades below their group's maximum (the last histogram bin
+    # and the floor term), groups that are all zero, a single nonzero element,
+    # and depths that leave a ragged last group; rows of A and columns of B,
+    # both tiles.  The line exponents differ from line to line, so a wrong
+    # exponent anywhere changes the result's bits.
+    r = np.random.default_rng(nmod)
+    m, n, k = 70, 66, 1000 + 7
+    announce(f"split mod per-line exponents moduli={nmod}")
+
+    def lines(count, depth):
+        X = r.uniform(-1, 1, (count, depth))
+        for i in range(count):
+            kind = i % 7
+            if kind == 1:
+                X[i] *= 2.0 ** -1060                     # subnormal line
+            elif kind == 2:
+                X[i, ::3] *= 2.0 ** -1060                # mixed normal / subnormal
+            elif kind == 3:
+                g = 16 * r.integers(0, depth // 16)
+                X[i, g] = 1.5
+                X[i, g + 1:g + 4] = [2.0 ** -11, 2.0 ** -12, 2.0 ** -13]   # bins 11, 12 and the floor
+            elif kind == 4:
+                X[i, : depth // 2] = 0.0                  # all-zero groups
+            elif kind == 5:
+                X[i] = 0.0
+                X[i, r.integers(0, depth)] = -3.0         # one nonzero element
+            elif kind == 6:
+                X[i] *= np.ldexp(1.0, r.integers(-30, 31, depth))
+        return X
+
+    A = lines(m, k)
+    B = lines(n, k).T.copy()
+    ba = orc.crt_line_betas(A, nmod, axis=1)
+    bb = orc.crt_line_betas(B, nmod, axis=0)
+    if orc.crt_beta(nmod, k) < 53:      # 15 moduli at this depth: every line at the cap
+        assert len(set(ba.tolist())) > 1 and len(set(bb.tolist())) > 1
+    want = orc.crt_gemm(A, B, nmod)
+    for tile in (1, 2):
+        assert _same(_mod(mgx, A, B, nmod, tile=tile), want), tile
+
+
 def test_large_magnitude_products(mgx, orc):
     # all-equal operands drive C' to k 2^(2 beta - 2), near the capacity bound,
     # and negative ones exercise the signed half of the range
@@ -147,6 +192,10 @@ def test_residues_at_the_top_of_the_operand_range(mgx, orc, k, beta):
     A = Ai.astype(np.float64)
     B = Bi.astype(np.float64)
     assert np.array_equal(A.astype(np.int64), Ai) and np.array_equal(B.astype(np.int64), Bi)
+    # every entry sits in the top binade of its line, so every line gets the
+    # dense-line exponent and A' = A, B' = B
+    assert set(orc.crt_line_betas(A, 15, axis=1).tolist()) == {beta}
+    assert set(orc.crt_line_betas(B, 15, axis=0).tolist()) == {beta}
     assert _same(_mod(mgx, A, B, 15), orc.crt_gemm(A, B, 15))
 
 

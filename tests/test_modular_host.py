@@ -2,8 +2,9 @@
 the host with g++ and checked against exact 128-bit integers
 (tests/modular_host_check.cpp): Chinese-remainder reconstruction at every
 scale, next to +-P/2 where the quotient estimate can miss, every
-representative for two moduli, exact rounding ties up to 2^117, and the
-general and narrow residue extraction.  The same header is compiled into
+representative for two moduli, exact rounding ties up to 2^117, the
+general and narrow residue extraction, and the per-line exponent search
+against its definition.  The same header is compiled into
 the GPU kernels; the GPU parity tests compare those against the oracle."""
 import re
 import shutil
@@ -23,6 +24,7 @@ def test_modular_arithmetic_against_exact_integers(tmp_path):
                    check=True)
     run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert run.returncode == 0, run.stdout + run.stderr
-    m = re.search(r"reconstructions (\d+) residues (\d+) bad 0", run.stdout)
+    m = re.search(r"reconstructions (\d+) residues (\d+) line exponents (\d+) bad 0", run.stdout)
     assert m, run.stdout
     assert int(m.group(1)) > 1_000_000 and int(m.group(2)) > 5_000_000
+    assert int(m.group(3)) > 100_000
