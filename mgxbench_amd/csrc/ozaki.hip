@@ -184,16 +184,17 @@ __global__ __launch_bounds__(256) void row_scan_kernel(const double *__restrict_
 }
 
 // Column scan of B (k x n, ldb): one lane per column, grid.y splits k in
-// segments of 256 rows, read 16 rows (one group) at a time.
+// segments of `seg` rows (a multiple of 16, chosen by the host so that short
+// depths still give a full grid), read 16 rows (one group) at a time.
 template <bool SUMS>
 __global__ __launch_bounds__(256) void col_scan_kernel(const double *__restrict__ B, int64_t k,
-                                                       int64_t n, int64_t ldb,
+                                                       int64_t n, int64_t ldb, int64_t seg,
                                                        unsigned long long *colmax,
                                                        unsigned long long *colsum) {
     const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (c >= n) return;
-    const int64_t k0 = (int64_t)blockIdx.y * 256;
-    const int64_t k1 = min(k, k0 + 256);
+    const int64_t k0 = (int64_t)blockIdx.y * seg;
+    const int64_t k1 = min(k, k0 + seg);
     unsigned mx = 0;
     unsigned long long ns = 0;
     for (int64_t kq = k0; kq < k1; kq += 16) {
@@ -1190,7 +1191,7 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
     MGX_ARG_CHECK(L.Mp / 64 <= 65535 && L.KTC <= 0x7fffffffll, "m or k too large for the row pass");
     MGX_ARG_CHECK(L.Kp / KT <= 65535 && L.Np / 64 <= 0x7fffffffll,
                   "k or n too large for the column pass");
-    MGX_ARG_CHECK((k + 255) / 256 <= 65535, "k too large for the column scan");
+    MGX_ARG_CHECK((k + 255) / 256 <= 65535, "k too large for the column scan");   // seg >= 256 then
 
     char *ws = (char *)workspace;
     // marks the library scratch as used by `st` once this call's work is queued
@@ -1246,14 +1247,18 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
         if ((rc = mgx::launch_status("row_scan_kernel")) != MGX_OK) return rc;
     }
     {
-        const unsigned gx = (unsigned)((n + 255) / 256);
-        const int64_t gy = (k + 255) / 256;
+        // segments of 256 rows, shorter (down to one 16-row group) when that
+        // leaves fewer than 2048 workgroups
+        const int64_t gx = (n + 255) / 256;
+        int64_t seg = 256;
+        while (seg > 16 && gx * ((k + seg - 1) / seg) < 2048) seg /= 2;
+        const int64_t gy = (k + seg - 1) / seg;
         if (mod)
-            hipLaunchKernelGGL(col_scan_kernel<true>, dim3(gx, (unsigned)gy), dim3(256), 0, st, B,
-                               k, n, ldb, cmax, csum);
+            hipLaunchKernelGGL(col_scan_kernel<true>, dim3((unsigned)gx, (unsigned)gy), dim3(256),
+                               0, st, B, k, n, ldb, seg, cmax, csum);
         else
-            hipLaunchKernelGGL(col_scan_kernel<false>, dim3(gx, (unsigned)gy), dim3(256), 0, st, B,
-                               k, n, ldb, cmax, csum);
+            hipLaunchKernelGGL(col_scan_kernel<false>, dim3((unsigned)gx, (unsigned)gy), dim3(256),
+                               0, st, B, k, n, ldb, seg, cmax, csum);
         if ((rc = mgx::launch_status("col_scan_kernel")) != MGX_OK) return rc;
     }
     if (mod) {
