@@ -68,7 +68,8 @@ PANELS = (128, 256, 512, 1024)
 SLICES = (8,)
 MODULI = (15, 16)
 # quantisation to >= 51 bits per row / column keeps the modular form within
-# plain FP64's normwise and componentwise error on stress inputs
+# plain FP64's normwise and componentwise error on stress inputs (the
+# guaranteed per-line exponent; spread-out lines get more)
 MIN_BETA = 51
 TILES = (_lib.TILE_LARGE, _lib.TILE_SMALL)
 STREAM_GROUPS = (1024, 4096)
