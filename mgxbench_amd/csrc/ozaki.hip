@@ -489,7 +489,7 @@ __global__ __launch_bounds__(256) void operand_tile_kernel(const double *__restr
 // scheme II: one group per modulus), K chunk by K chunk, pair by pair, k tile
 // by k tile.  A chunk is a whole even number of k tiles.
 
-constexpr int TGROUP_M = 8;       // C block rows per raster group
+constexpr int TGROUP_M = 4;       // C block rows per raster group
 constexpr int CHUNK_GRAIN = 2;    // K chunks are whole multiples of this many 64-k tiles
 constexpr int NSTAGE = 5;         // LDS ring depth (stages)
 
