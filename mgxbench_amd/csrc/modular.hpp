@@ -211,12 +211,17 @@ constexpr CrtTables make_crt_tables() {
     return T;
 }
 
-// Carries the integer-valued limbs x[0..2] into [0, 2^40) (floor division by
-// 2^40); the value sum x_j 2^(40 j) is unchanged and x[3] keeps its sign.
-// Exact while every |x_j| stays below 2^53 - 2^14.
-__host__ __device__ inline void normalize40(double (&x)[4]) {
+// Limbs a Chinese-remainder value of n moduli needs: P_n < 2^120 for n <= 15
+// (P_15 < 2^118), so three 40-bit limbs hold it; P_16 needs four.
+__host__ __device__ constexpr int crt_limbs(int n) { return n <= 15 ? 3 : 4; }
+
+// Carries the integer-valued limbs x[0..L-2] into [0, 2^40) (floor division
+// by 2^40); the value sum x_j 2^(40 j) is unchanged and the top limb keeps
+// its sign.  Exact while every |x_j| stays below 2^53 - 2^14.
+template <int L>
+__host__ __device__ inline void normalize40(double (&x)[L]) {
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
+    for (int j = 0; j + 1 < L; ++j) {
         const double c = floor(x[j] * 0x1p-40);
         x[j] = fma(-c, 0x1p40, x[j]);
         x[j + 1] += c;
@@ -224,62 +229,76 @@ __host__ __device__ inline void normalize40(double (&x)[4]) {
 }
 
 // Exact integer x with x = r_l (mod m_l), -P/2 < x <= P/2, from residues
-// r_l in [0, m_l), l < n, rounded to double (T = make_crt_tables()).
+// r_l in [0, m_l), l < n, rounded to double (T = make_crt_tables(),
+// L = crt_limbs(n); W_l < P_n, so limbs of W at and above L are zero).
 //   S_j = sum_l r_l W_lj          exact: every partial sum < 16 * 255 * 2^40 < 2^52
 //   q = rint(S / P) from a double estimate (error < 2^-40), so that
-//   x = S - q P (limb by limb, exact) is the symmetric representative unless
-//   |x| lies within 2^-39 P of P/2; that case is recognised from |x| >= the
-//   threshold and settled exactly.  |x| then goes to double with one rounding.
-__host__ __device__ inline double reconstruct(const CrtTables &T, const uint32_t (&r)[MAX_MODULI],
-                                              int n) {
+//   x = S - q P (limb by limb, exact; after the carries the low limbs lie in
+//   [0, 2^40) and the top limb holds the sign) is the symmetric
+//   representative unless |x| lies within 2^-39 P of P/2; that case is
+//   recognised from the estimate |x| >= the threshold and settled exactly.
+// The signed x then goes to double with one rounding: x = a + b + c with
+// a = top limbs 2^80 (signed), b = x_1 2^40 and c = x_0 (both in [0, 2^80)
+// and [0, 2^40)), each exact.  s + e = a + b exactly (fast two-sum: a = 0 or
+// |a| >= 2^80 > b).  e = 0: RN(s + c) is the one rounding.  e != 0: a + b is
+// a multiple of 2^40 that needs more than 53 bits, so |a + b| > 2^93, where
+// every double and every rounding midpoint is a multiple of 2^40; x lies in
+// [a + b, a + b + 2^40) and any c in (0, 2^40) rounds like 2^39.  e + 2^39
+// is exact (e is a multiple of 2^40 below 2^67), so RN(s + (e + sticky)) =
+// RN(x), for either sign of x.
+template <int L>
+__host__ __device__ inline double reconstruct_limbs(const CrtTables &T,
+                                                    const uint32_t (&r)[MAX_MODULI], int n) {
+    static_assert(L == 3 || L == 4, "three or four limbs");
     const double(&W)[MAX_MODULI][4] = T.w[n];
     const double(&P)[4] = T.p[n];
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    double x[L];
+#pragma unroll
+    for (int j = 0; j < L; ++j) x[j] = 0.0;
 #pragma unroll
     for (int l = 0; l < MAX_MODULI; ++l) {
         const double rl = l < n ? (double)(r[l] & 0xffu) : 0.0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) s[j] = fma(rl, W[l][j], s[j]);
+        for (int j = 0; j < L; ++j) x[j] = fma(rl, W[l][j], x[j]);
     }
-    const double sd = fma(fma(fma(s[3], 0x1p40, s[2]), 0x1p40, s[1]), 0x1p40, s[0]);
+    double sd = x[L - 1];
+#pragma unroll
+    for (int j = L - 2; j >= 0; --j) sd = fma(sd, 0x1p40, x[j]);
     const double q = rint(sd * T.inv_p[n]);   // 0 <= q <= 4080
-    double x[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) x[j] = fma(-q, P[j], s[j]);   // |x_j| < 2^52
-    normalize40(x);
-    // x < 0 iff its top limb is; negate limb by limb and carry again: |x|
-    double sg = x[3] < 0.0 ? -1.0 : 1.0;
+    for (int j = 0; j < L; ++j) x[j] = fma(-q, P[j], x[j]);   // |x_j| < 2^52
+    normalize40<L>(x);
+    const double hi = L == 4 ? fma(x[L - 1], 0x1p40, x[L - 2]) : x[L - 1];   // |hi| < 2^48
+    const double est = fma(fma(hi, 0x1p40, x[1]), 0x1p40, x[0]);
+    if (fabs(est) >= T.half_p_lo[n]) {
+        // sign of x known from the estimate; t = 2x - sg P, exactly: x leaves
+        // (-P/2, P/2] when t > 0 (x > 0) or t <= 0 (x < 0), and moves by -sg P
+        const double sg = est < 0.0 ? -1.0 : 1.0;
+        double t[L];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) x[j] *= sg;
-    normalize40(x);
-    const double mag = fma(fma(fma(x[3], 0x1p40, x[2]), 0x1p40, x[1]), 0x1p40, x[0]);
-    if (mag >= T.half_p_lo[n]) {
-        // t = 2|x| - P, exactly; |x| > P/2 (or x = -P/2) takes the other representative
-        double t[4];
+        for (int j = 0; j < L; ++j) t[j] = fma(2.0, x[j], -sg * P[j]);
+        normalize40<L>(t);
+        bool zero = true;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) t[j] = fma(2.0, x[j], -P[j]);
-        normalize40(t);
-        const bool zero = t[0] == 0.0 && t[1] == 0.0 && t[2] == 0.0 && t[3] == 0.0;
-        if ((t[3] >= 0.0 && !zero) || (zero && sg < 0.0)) {
+        for (int j = 0; j < L; ++j) zero = zero && t[j] == 0.0;
+        const bool out = sg > 0.0 ? (t[L - 1] >= 0.0 && !zero) : (t[L - 1] < 0.0 || zero);
+        if (out) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) x[j] = P[j] - x[j];
-            normalize40(x);
-            sg = -sg;
+            for (int j = 0; j < L; ++j) x[j] = fma(-sg, P[j], x[j]);
+            normalize40<L>(x);
         }
     }
-    // |x| = a + b + c with a = (x3 2^40 + x2) 2^80, b = x1 2^40, c = x0, each
-    // exact (x3 2^40 + x2 < 2^48, limbs in [0, 2^40)).  s + e = a + b exactly
-    // (fast two-sum: a = 0 or a > b).  e = 0: RN(s + c) is the one rounding.
-    // e != 0: a + b >= 2^93, so the result's half-ulp is a multiple of 2^40
-    // and any c in (0, 2^40) rounds like 2^39; e + 2^39 is exact (e is a
-    // multiple of 2^40 below 2^67), so RN(s + (e + sticky)) = RN(|x|).
-    const double a = fma(x[3], 0x1p40, x[2]) * 0x1p80;
+    const double a = (L == 4 ? fma(x[L - 1], 0x1p40, x[L - 2]) : x[L - 1]) * 0x1p80;
     const double b = x[1] * 0x1p40;
     const double sum = a + b;
     const double e = b - (sum - a);
     const double c = e == 0.0 ? x[0] : e + (x[0] != 0.0 ? 0x1p39 : 0.0);
-    const double d = sum + c;
-    return sg < 0.0 ? -d : d;
+    return sum + c;
+}
+
+__host__ __device__ inline double reconstruct(const CrtTables &T, const uint32_t (&r)[MAX_MODULI],
+                                              int n) {
+    return crt_limbs(n) == 3 ? reconstruct_limbs<3>(T, r, n) : reconstruct_limbs<4>(T, r, n);
 }
 
 }  // namespace modular

@@ -991,13 +991,12 @@ __device__ __forceinline__ double crt_value(const uint32_t (&res)[MAX_MODULI], i
     }
 }
 
-template <int TB, int WGM, int WGN, int NM>
+template <int TB, int WGM, int WGN, int NM, bool ACC>
 __global__ __launch_bounds__(256) void reconstruct_kernel(const uint8_t *__restrict__ R,
                                                           const int32_t *__restrict__ rshift,
                                                           const int32_t *__restrict__ cshift,
                                                           double *__restrict__ C, int64_t M,
-                                                          int64_t N, int64_t ldc, int nmod,
-                                                          int accumulate) {
+                                                          int64_t N, int64_t ldc, int nmod) {
     using S = TileShape<TB, TB, WGM, WGN>;
     const int nm = NM > 0 ? NM : nmod;
     const int tiles_m = (int)((M + TB - 1) / TB);
@@ -1013,33 +1012,47 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(const uint8_t *__restr
     const int64_t n0 = (tile % tiles_n) * TB;
     const int64_t row0 = m0 + (wave / WGN) * S::WTM + 4 * (lane >> 4);
     const int64_t col0 = n0 + (wave % WGN) * S::WTN + (lane & 15);
-    {
-        v4i runs[MAX_MODULI];
+    // A lane's 16 outputs are fragments f = 4q + w (w = 0..3) at rows
+    // 16 (f / FJ) + r and columns 16 (f % FJ): with FJ a multiple of 4 that
+    // is 4 rows (r) by 4 columns (w).  Every load (residue runs, line shifts,
+    // the accumulated C) is issued before the arithmetic.
+    static_assert(S::FJ % 4 == 0, "a residue run covers 4 columns of one fragment row");
+    const int64_t rbase = row0 + 16 * ((4 * q) / S::FJ);
+    const int64_t cbase = col0 + 16 * ((4 * q) % S::FJ);
+    v4i runs[MAX_MODULI];
 #pragma unroll
-        for (int l = 0; l < MAX_MODULI; ++l)
-            if (l < nm)
-                runs[l] = *(const v4i *)(R + residue_run(l, tiles, tile, S::NW, wave, S::QPL, q) +
-                                         lane * 16);
+    for (int l = 0; l < MAX_MODULI; ++l)
+        if (l < nm)
+            runs[l] = *(const v4i *)(R + residue_run(l, tiles, tile, S::NW, wave, S::QPL, q) +
+                                     lane * 16);
+    int er[4], ec[4];
 #pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            const int f = 4 * q + w;
-            const int64_t col = col0 + 16 * (f % S::FJ);
-            const int ec = col < N ? cshift[col] : 0;
+    for (int r = 0; r < 4; ++r) er[r] = rbase + r < M ? rshift[rbase + r] : 0;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t row = row0 + 16 * (f / S::FJ) + r;
-                if (row < M && col < N) {
-                    uint32_t res[MAX_MODULI];
+    for (int w = 0; w < 4; ++w) ec[w] = cbase + 16 * w < N ? cshift[cbase + 16 * w] : 0;
+    double cin[4][4];   // the C being accumulated into (ACC only)
 #pragma unroll
-                    for (int l = 0; l < MAX_MODULI; ++l)
-                        res[l] = l < nm ? ((uint32_t)runs[l][w] >> (8 * r)) & 0xffu : 0u;
-                    const int er = rshift[row];
-                    const double x = (er == EXP_BAD || ec == EXP_BAD)
-                                         ? __longlong_as_double(0x7ff8000000000000ll)
-                                         : ldexp(crt_value<NM>(res, nm), er + ec);
-                    double *c = C + row * ldc + col;
-                    *c = accumulate ? *c + x : x;
-                }
+    for (int w = 0; w < 4; ++w)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            cin[w][r] = ACC && rbase + r < M && cbase + 16 * w < N
+                            ? C[(rbase + r) * ldc + cbase + 16 * w]
+                            : 0.0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const int64_t col = cbase + 16 * w;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t row = rbase + r;
+            if (row < M && col < N) {
+                uint32_t res[MAX_MODULI];
+#pragma unroll
+                for (int l = 0; l < MAX_MODULI; ++l)
+                    res[l] = l < nm ? ((uint32_t)runs[l][w] >> (8 * r)) & 0xffu : 0u;
+                const double x = (er[r] == EXP_BAD || ec[w] == EXP_BAD)
+                                     ? __longlong_as_double(0x7ff8000000000000ll)
+                                     : ldexp(crt_value<NM>(res, nm), er[r] + ec[w]);
+                C[row * ldc + col] = ACC ? cin[w][r] + x : x;
             }
         }
     }
@@ -1345,9 +1358,15 @@ static int split_gemm_impl(int form, int slices, int tile, const double *A, cons
         // the common modulus counts get compile-time specialisations
         auto launch = [&](auto tb, auto nm) {
             constexpr int TBv = decltype(tb)::value, NMv = decltype(nm)::value;
-            hipLaunchKernelGGL((reconstruct_kernel<TBv, 2, TBv == 256 ? 4 : 2, NMv>),
-                               dim3((unsigned)blocks), dim3(256), 0, st, Rres, rshift, cshift, C,
-                               m, n, ldc, slices, accumulate ? 1 : 0);
+            constexpr int WGNv = TBv == 256 ? 4 : 2;
+            if (accumulate)
+                hipLaunchKernelGGL((reconstruct_kernel<TBv, 2, WGNv, NMv, true>),
+                                   dim3((unsigned)blocks), dim3(256), 0, st, Rres, rshift, cshift,
+                                   C, m, n, ldc, slices);
+            else
+                hipLaunchKernelGGL((reconstruct_kernel<TBv, 2, WGNv, NMv, false>),
+                                   dim3((unsigned)blocks), dim3(256), 0, st, Rres, rshift, cshift,
+                                   C, m, n, ldc, slices);
         };
         using L = std::integral_constant<int, 256>;
         using Sm = std::integral_constant<int, 128>;
