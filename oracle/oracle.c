@@ -32,11 +32,16 @@
  *    integer product in 128-bit arithmetic and one correctly rounded
  *    conversion per output.
  *
- * Parity pins (tests/test_oracle.py): exact rational brute force on tiny
+ * Parity pins.  tests/test_oracle.py: exact rational brute force on tiny
  * inputs for the fma chains, the accumulate form, the double-double dot and
  * the split reconstruction; the A=1/B=2 known answer; integer-valued inputs
  * (exact in every order); transposition identity; the splitmix64 published
- * first outputs for seed 0; closed-form validation cases.
+ * first outputs for seed 0; closed-form validation cases; the MLP against
+ * finite differences and a closed form.  tests/test_oracle_crt.py: the
+ * orc_crt_* model against Python arbitrary-precision integers, the scaling
+ * rule's inequality in exact rationals and the classical exponent on dense
+ * lines.  tests/test_oracle_generators.py: the stress-input generators
+ * (exponent range, exact power-of-two scaling) in closed form.
  */
 #include <math.h>
 #include <stdint.h>
