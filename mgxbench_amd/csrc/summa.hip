@@ -24,10 +24,11 @@
 //  * pinned: each group arrives as one A copy (a strided 2D copy of the
 //    group's columns) and one B copy, gathered by the DMA engine;
 //  * pageable (the reference's case, pack_a / pack_b,
-//    2D-block-matrix/mmb.cu:60-72, 264-265): whole panels of up to
-//    PACK_UNIT_K columns are packed into one of two pinned staging buffers by
-//    a pool of host threads, then copied; packing the next unit overlaps the
-//    copy of the current one.
+//    2D-block-matrix/mmb.cu:60-72, 264-265): units of PACK_UNIT_K columns
+//    (a group's last unit narrower; a unit may be part of a panel) are packed
+//    into one of two pinned staging buffers by a pool of host threads, then
+//    copied into their place in the group's device buffer; packing the next
+//    unit overlaps the copy of the current one.
 // With host-synchronous copies (the CCC/XXX/WWW/SSS codes) or `serialize`,
 // panels move one at a time and the host waits for each copy, as the
 // reference's P=1 paths do.
@@ -50,9 +51,11 @@
 
 namespace {
 
-// Columns packed per pinned staging unit with pageable strips (whole panels;
-// a single wider panel is its own unit).
-constexpr int64_t PACK_UNIT_K = 1024;
+// Columns packed per pinned staging unit with pageable strips.  Measured on
+// the 2x2 share at N = 16384 (16 host threads): units of 256 columns take
+// 44-46 ms per loop at panel widths 256, 512 and 1024, against 52-57 ms with
+// units of 1024 columns and 47 ms with units of 64.
+constexpr int64_t PACK_UNIT_K = 256;
 constexpr int MAX_HOST_THREADS = 64;
 
 bool form_known(int form) {
@@ -264,9 +267,9 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
     const bool panelwise = P->copy_mode == MGX_SUMMA_COPY_HOST_SYNC || P->serialize;
     MGX_ARG_CHECK(!graph || (pinned && !panelwise),
                   "hipGraph capture needs pinned host strips, async copies and no serialization");
-    // pageable strips: staging units of whole panels (one panel when the host
-    // waits for every copy)
-    const int64_t unit = panelwise ? panel : std::max(panel, PACK_UNIT_K / panel * panel);
+    // pageable strips: staging units of PACK_UNIT_K columns (one panel when
+    // the host waits for every copy)
+    const int64_t unit = panelwise ? panel : PACK_UNIT_K;
 
     SummaResources R;
     MGX_HIP_TRY(hipStreamCreateWithFlags(&R.compute, hipStreamNonBlocking));
