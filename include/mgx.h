@@ -246,9 +246,10 @@ int mgx_error_report(const double *A, const double *B, const double *C,
  * Host strips: pinned (mgx_host_alloc) strips are gathered directly by
  * strided DMA, one A copy and one B copy per group; pageable strips are
  * packed (the reference's pack_a / pack_b) by a pool of host_threads host
- * threads into two library-owned pinned staging buffers in units of 256
+ * threads into two library-owned pinned staging buffers in units of 512
  * columns (parts of wider panels), packing the next unit while the current
- * one is copied into its place in the group's device buffer.  dC is a device m x n matrix that is accumulated into (zero it for
+ * one is copied into its place in the group's device buffer; the staging
+ * buffers are written once before the timed loop.  dC is a device m x n matrix that is accumulated into (zero it for
  * C = A.B).  Synchronous: all resources (buffers, streams, host threads) are
  * created before and released after the timed loop.
  * Consecutive panels form groups (mgx_summa_groups) and each group is one

