@@ -843,11 +843,17 @@ __device__ __forceinline__ void product_body(const ProductArgs &a) {
             fb[j] = *(const v4i *)(lb + r * KT + ((lk ^ swz(r)) << 4));
         }
     };
+    // Serpentine order (j descending on odd i): consecutive MFMAs share one
+    // operand, so only one operand's bits change per instruction.  The part
+    // is power-bound under these products; against the row-major order this
+    // measured 0.6-1.0% faster (same results: each accumulator sees the same
+    // sequence).
     auto mfma_half = [&](int i0, const v4i (&fa)[FH], const v4i (&fb)[FJ]) {
 #pragma unroll
         for (int i = 0; i < FH; ++i)
 #pragma unroll
-            for (int j = 0; j < FJ; ++j) {
+            for (int jj = 0; jj < FJ; ++jj) {
+                const int j = (i & 1) ? FJ - 1 - jj : jj;
                 if constexpr (BF)
                     accf[i0 + i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                         __builtin_bit_cast(v8bf, fa[i]), __builtin_bit_cast(v8bf, fb[j]),
