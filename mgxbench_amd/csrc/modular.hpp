@@ -178,8 +178,9 @@ __host__ __device__ inline int residue_int8_narrow(double x, int l) {
 
 // Direct Chinese remaindering tables for n = 2..16 moduli, in FP64:
 //   x = (sum_l r_l W_l) mod P_n,   W_l = (P_n / m_l) ((P_n / m_l)^-1 mod m_l) < P_n,
-// with W and P_n in four 40-bit limbs held exactly as doubles, fl(1/P_n),
-// and a threshold just below P_n / 2 for the exact range check.
+// with W and P_n in four 40-bit limbs held exactly as doubles (the fourth is
+// zero up to 15 moduli), fl(1/P_n), and a threshold just below P_n / 2 for
+// the exact range check.
 struct CrtTables {
     double w[MAX_MODULI + 1][MAX_MODULI][4];
     double p[MAX_MODULI + 1][4];

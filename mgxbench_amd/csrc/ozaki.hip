@@ -987,6 +987,7 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINB) void bf16_product_kernel(Prod
 // n residues, recovers C' exactly, rounds once and scales.
 // NM > 0 fixes the modulus count at compile time (the table words then fold
 // into immediates and the unused moduli drop out); NM = 0 reads it from nmod.
+// ACC adds the result to C (read before the arithmetic) instead of storing it.
 template <int NM>
 __device__ __forceinline__ double crt_value(const uint32_t (&res)[MAX_MODULI], int nmod) {
     if constexpr (NM > 0) {
