@@ -106,8 +106,10 @@ def test_large_magnitude_products(mgx, orc):
     A = np.ones((m, k))
     B = -np.ones((k, n))
     A[::2] = -1.0
-    for nmod in (2, 9, 16):
-        assert _same(_mod(mgx, A, B, nmod), orc.crt_gemm(A, B, nmod))
+    # 2 and 9 moduli run the kernel with a run-time modulus count, 14 and 15
+    # its three-limb compile-time forms, 16 the four-limb one
+    for nmod in (2, 9, 14, 15, 16):
+        assert _same(_mod(mgx, A, B, nmod), orc.crt_gemm(A, B, nmod)), nmod
 
 
 def test_accumulate_form(mgx, orc):
