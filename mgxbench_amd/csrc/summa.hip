@@ -267,9 +267,9 @@ extern "C" int mgx_summa_run(const mgx_summa_params *P, const double *hA, int64_
     const bool panelwise = P->copy_mode == MGX_SUMMA_COPY_HOST_SYNC || P->serialize;
     MGX_ARG_CHECK(!graph || (pinned && !panelwise),
                   "hipGraph capture needs pinned host strips, async copies and no serialization");
-    // pageable strips: staging units of PACK_UNIT_K columns (one panel when
-    // the host waits for every copy)
-    const int64_t unit = panelwise ? panel : PACK_UNIT_K;
+    // pageable strips: staging units of PACK_UNIT_K columns, at most the
+    // widest group (one panel when the host waits for every copy)
+    const int64_t unit = panelwise ? panel : std::min(PACK_UNIT_K, gmax);
 
     SummaResources R;
     MGX_HIP_TRY(hipStreamCreateWithFlags(&R.compute, hipStreamNonBlocking));
