@@ -42,6 +42,9 @@ def parse_segment(seg: list[str]) -> tuple[int, list[str]]:
 
 def main(argv: list[str] | None = None) -> int:
     argv = sys.argv[1:] if argv is None else argv
+    if argv in (["-h"], ["--help"]):
+        print(__doc__.strip())
+        return 0
     segments = [parse_segment(s) for s in split_segments(argv)]
     total = sum(n for n, _ in segments)
     if not segments or not segments[0][1]:

@@ -82,6 +82,12 @@ def test_launch_runs_one_rank_and_refuses_more(tmp_path):
     assert rc == 7
     assert launch.main(["-np", "1", "x", ":", "-np", "1", "y"]) == 2
     assert launch.main(["-np", "2", "x"]) == 2
+    assert launch.main([]) == 2
+
+
+def test_launch_help(capsys):
+    assert launch.main(["--help"]) == 0
+    assert "-np 1" in capsys.readouterr().out
 
 
 # ---- end-to-end runner with a simulated launcher ------------------------------------
